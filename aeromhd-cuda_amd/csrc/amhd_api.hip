@@ -1,0 +1,639 @@
+// amhd_api.hip -- the C ABI of libaeromhd_hip.so (declared in
+// include/aeromhd.h).  Built with -ffp-contract=off: the standalone FP64
+// operators here are the reference's IEEE arithmetic.
+#include <algorithm>
+#include <cstdarg>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+#include <cstdio>
+#include <cstring>
+#include <cmath>
+
+#include "aeromhd.h"
+#define AMHD_API_TU
+#include "amhd_launch.hpp"
+
+using namespace amhd;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                          \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess) return fail(-2, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                                          __FILE__, __LINE__);                                 \
+    } while (0)
+
+// Per-device scratch for the context-free operators (smax bits, CFL
+// partials, a StepCtl).  Allocated once per device, freed at exit.
+constexpr int SCRATCH_PARTS = 1024;
+struct Scratch {
+    double* part = nullptr;               // SCRATCH_PARTS * NPART_DT
+    StepCtl* ctl = nullptr;
+    unsigned long long* bits = nullptr;
+};
+std::mutex g_scratch_mu;
+std::unordered_map<int, Scratch> g_scratch;
+
+int get_scratch(Scratch** out) {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto it = g_scratch.find(dev);
+    if (it == g_scratch.end()) {
+        Scratch s;
+        HIP_TRY(hipMalloc(&s.part, sizeof(double) * SCRATCH_PARTS * NPART_DT));
+        HIP_TRY(hipMalloc(&s.ctl, sizeof(StepCtl)));
+        HIP_TRY(hipMalloc(&s.bits, sizeof(unsigned long long)));
+        it = g_scratch.emplace(dev, s).first;
+    }
+    *out = &it->second;
+    return 0;
+}
+
+struct Ctx {
+    int nx = 0, ny = 0, dtype = AMHD_F64;
+    amhd_params p{};
+    int rows = 16, n_strips = 0, n_segs = 0;
+    long plane = 0;
+    int pitch = 0;
+    void* buf[3] = {nullptr, nullptr, nullptr};
+    int cur = 0;
+    double* staging = nullptr;   // ghosted f64, (10, nx+4, ny+4)
+    double* part_dt = nullptr;
+    int part_cap = 0, n_part = 0;
+    bool part_valid = false;
+    double* part_bf = nullptr;
+    double* gsum = nullptr;      // global-sum partials
+    StepCtl* ctl = nullptr;
+    hipStream_t stream = nullptr;
+    hipGraphExec_t graph[3] = {nullptr, nullptr, nullptr};
+    int graph_rows = -1;
+};
+
+size_t esize(int dtype) { return dtype == AMHD_F32 ? sizeof(float) : sizeof(double); }
+
+void set_tiling(Ctx* c, int rows) {
+    if (rows <= 0) rows = 16;
+    if (rows > c->nx) rows = c->nx;
+    c->rows = rows;
+    c->n_strips = (c->ny + STRIP - 1) / STRIP;
+    c->n_segs = (c->nx + rows - 1) / rows;
+}
+
+int n_waves(const Ctx* c) { return c->n_strips * c->n_segs; }
+
+template <class real>
+StageParams<real> stage_args(const Ctx* c, int stage, const real* u0, const real* us, real* uo) {
+    StageParams<real> a{};
+    a.u0 = u0; a.us = us; a.uo = uo;
+    a.plane = c->plane; a.oplane = c->plane;
+    a.pitch = c->pitch; a.opitch = c->pitch;
+    a.nx = c->nx; a.ny = c->ny;
+    a.bcx_periodic = c->p.bc_x_periodic; a.bcy_periodic = c->p.bc_y_periodic;
+    a.n_strips = c->n_strips; a.n_segs = c->n_segs; a.rows = c->rows;
+    a.gamma = c->p.gamma; a.dx = c->p.dx; a.dy = c->p.dy;
+    a.ctl = nullptr;
+    a.part_dt = stage == 3 ? c->part_dt : nullptr;
+    a.part_bf = stage == 1 ? c->part_bf : nullptr;
+    return a;
+}
+
+hipError_t launch_stage_any(const Ctx* c, int stage, const void* u0, const void* us, void* uo, double dt, double ch,
+                            double decay, const StepCtl* ctl) {
+    if (c->dtype == AMHD_F64) {
+        auto a = stage_args<double>(c, stage, (const double*)u0, (const double*)us, (double*)uo);
+        a.dt = dt; a.ch = ch; a.decay = decay; a.ctl = ctl;
+        return launch_stage_f64(stage, c->p.powell != 0, a, c->stream);
+    }
+    auto a = stage_args<float>(c, stage, (const float*)u0, (const float*)us, (float*)uo);
+    a.dt = dt; a.ch = ch; a.decay = decay; a.ctl = ctl;
+    return launch_stage_f32(stage, c->p.powell != 0, a, c->stream);
+}
+
+// One SSP-RK3 step on buffers rotating cur -> cur+1 (mod 3).
+hipError_t enqueue_step(Ctx* c, int cur, double dt, double ch, double decay, const StepCtl* ctl) {
+    void* u0 = c->buf[cur];
+    void* u1 = c->buf[(cur + 1) % 3];
+    void* u2 = c->buf[(cur + 2) % 3];
+    hipError_t e;
+    if ((e = launch_stage_any(c, 1, u0, u0, u1, dt, ch, decay, ctl)) != hipSuccess) return e;
+    if ((e = launch_stage_any(c, 2, u0, u1, u2, dt, ch, decay, ctl)) != hipSuccess) return e;
+    return launch_stage_any(c, 3, u0, u2, u1, dt, ch, decay, ctl);
+}
+
+hipError_t enqueue_ctl(Ctx* c, int n_part, double host_dt, double cfl, int mode, StepCtl* ctl) {
+    step_ctl_kernel<<<1, 256, 0, c->stream>>>(ctl, c->part_dt, n_part, c->p.bc_x_periodic ? nullptr : c->part_bf,
+                                               c->n_strips, c->p.dx, c->p.dy, host_dt, cfl, mode);
+    return hipGetLastError();
+}
+
+int ensure_partials(Ctx* c) {
+    if (c->part_valid) return 0;
+    int g;
+    if (c->dtype == AMHD_F64)
+        g = launch_dt_partials_f64((const double*)c->buf[c->cur], c->plane, c->pitch, c->nx, c->ny, c->p.gamma,
+                                   c->part_dt, c->part_cap, c->stream);
+    else
+        g = launch_dt_partials_f32((const float*)c->buf[c->cur], c->plane, c->pitch, c->nx, c->ny, c->p.gamma,
+                                   c->part_dt, c->part_cap, c->stream);
+    if (g < 0) return fail(-2, "dt partials launch failed: %s", hipGetErrorString(hipGetLastError()));
+    c->n_part = g;
+    c->part_valid = true;
+    return 0;
+}
+
+void drop_graphs(Ctx* c) {
+    for (auto& g : c->graph) {
+        if (g) hipGraphExecDestroy(g);
+        g = nullptr;
+    }
+}
+
+// FP64 global sums of the interior state (one partial per workgroup).
+template <class real>
+__global__ __launch_bounds__(256) void global_sums_kernel(const real* __restrict__ u, long plane, int pitch, int nx,
+                                                          int ny, double dx, double dy, double* part) {
+    double s_rho = 0.0, s_en = 0.0, m_div = 0.0, s_div2 = 0.0;
+    const long n = (long)nx * ny;
+    for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(k / ny), j = (int)(k - (long)i * ny);
+        const long o = (long)i * pitch + j;
+        s_rho += double(u[RHO * plane + o]);
+        s_en += double(u[EN * plane + o]);
+        if (i >= 1 && i < nx - 1 && j >= 1 && j < ny - 1) {
+            const real* bx = u + BX * plane;
+            const real* by = u + BY * plane;
+            const double d = (0.5 * (double(bx[o]) + double(bx[o + pitch])) -
+                              0.5 * (double(bx[o - pitch]) + double(bx[o]))) / dx +
+                             (0.5 * (double(by[o]) + double(by[o + 1])) - 0.5 * (double(by[o - 1]) + double(by[o]))) /
+                                 dy;
+            m_div = fmax(m_div, fabs(d));
+            s_div2 += d * d;
+        }
+    }
+    __shared__ double sh[4][4];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    s_rho = wave_sum(s_rho);
+    s_en = wave_sum(s_en);
+    s_div2 = wave_sum(s_div2);
+    m_div = wave_max(m_div);
+    if (lane == 0) { sh[0][w] = s_rho; sh[1][w] = s_en; sh[2][w] = m_div; sh[3][w] = s_div2; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0, b = 0, c = 0, d = 0;
+        for (int q = 0; q < (int)(blockDim.x >> 6); ++q) { a += sh[0][q]; b += sh[1][q]; c = fmax(c, sh[2][q]); d += sh[3][q]; }
+        part[blockIdx.x * 4 + 0] = a;
+        part[blockIdx.x * 4 + 1] = b;
+        part[blockIdx.x * 4 + 2] = c;
+        part[blockIdx.x * 4 + 3] = d;
+    }
+}
+
+constexpr int GSUM_BLOCKS = 512;
+
+}  // namespace
+
+extern "C" {
+
+const char* amhd_last_error(void) { return g_err.c_str(); }
+int amhd_abi_version(void) { return AMHD_ABI_VERSION; }
+
+int amhd_create(void** out, int nx, int ny, int dtype, const amhd_params* p) {
+    if (!out || !p) return fail(-1, "amhd_create: null argument");
+    *out = nullptr;
+    if (nx < 2 || ny < 2) return fail(-1, "amhd_create: grid %dx%d too small (need nx, ny >= 2)", nx, ny);
+    if (dtype != AMHD_F64 && dtype != AMHD_F32) return fail(-1, "amhd_create: unknown dtype %d", dtype);
+    if (!(p->gamma > 1.0) || !(p->dx > 0.0) || !(p->dy > 0.0))
+        return fail(-1, "amhd_create: need gamma > 1, dx > 0, dy > 0");
+    Ctx* c = new Ctx();
+    c->nx = nx; c->ny = ny; c->dtype = dtype; c->p = *p;
+    c->pitch = ny;
+    c->plane = (long)nx * ny;
+    set_tiling(c, 16);
+    const size_t sb = esize(dtype) * NV * (size_t)c->plane;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    for (int k = 0; k < 3 && e == hipSuccess; ++k) e = hipMalloc(&c->buf[k], sb);
+    if (e == hipSuccess) e = hipMalloc(&c->staging, sizeof(double) * NV * (size_t)(nx + 4) * (ny + 4));
+    // CFL partials: one per wavefront at any tiling (rows >= 1), or one per
+    // workgroup of the standalone partials kernel.
+    c->part_cap = std::max((int)((long)nx * ((ny + STRIP - 1) / STRIP)), 4096);
+    if (e == hipSuccess) e = hipMalloc(&c->part_dt, sizeof(double) * NPART_DT * (size_t)c->part_cap);
+    if (e == hipSuccess) e = hipMalloc(&c->part_bf, sizeof(double) * 2 * (size_t)c->n_strips);
+    if (e == hipSuccess) e = hipMalloc(&c->gsum, sizeof(double) * 4 * GSUM_BLOCKS);
+    if (e == hipSuccess) e = hipMalloc(&c->ctl, sizeof(StepCtl));
+    if (e == hipSuccess) {
+        StepCtl z{};
+        e = hipMemcpyAsync(c->ctl, &z, sizeof z, hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess) e = hipMemsetAsync(c->part_bf, 0, sizeof(double) * 2 * c->n_strips, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    }
+    if (e != hipSuccess) {
+        int rc = fail(e == hipErrorOutOfMemory ? -3 : -2, "amhd_create: %s", hipGetErrorString(e));
+        amhd_destroy(c);
+        return rc;
+    }
+    *out = c;
+    return 0;
+}
+
+int amhd_destroy(void* ctx) {
+    if (!ctx) return 0;
+    Ctx* c = (Ctx*)ctx;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    drop_graphs(c);
+    for (auto& b : c->buf) if (b) hipFree(b);
+    if (c->staging) hipFree(c->staging);
+    if (c->part_dt) hipFree(c->part_dt);
+    if (c->part_bf) hipFree(c->part_bf);
+    if (c->gsum) hipFree(c->gsum);
+    if (c->ctl) hipFree(c->ctl);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return 0;
+}
+
+int amhd_set_rows(void* ctx, int rows) {
+    if (!ctx) return fail(-1, "amhd_set_rows: null context");
+    if (rows < 0) return fail(-1, "amhd_set_rows: rows must be >= 0");
+    Ctx* c = (Ctx*)ctx;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    set_tiling(c, rows);
+    if (c->graph_rows != c->rows) drop_graphs(c);
+    c->part_valid = false;   // per-wave partials follow the tiling
+    return 0;
+}
+
+int amhd_get_rows(void* ctx) { return ctx ? ((Ctx*)ctx)->rows : -1; }
+
+int amhd_upload_state(void* ctx, const double* U, int on_device) {
+    if (!ctx || !U) return fail(-1, "amhd_upload_state: null argument");
+    Ctx* c = (Ctx*)ctx;
+    const size_t gbytes = sizeof(double) * NV * (size_t)(c->nx + 4) * (c->ny + 4);
+    HIP_TRY(hipMemcpyAsync(c->staging, U, gbytes, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                           c->stream));
+    hipError_t e = c->dtype == AMHD_F64
+                       ? launch_ingest_f64(c->staging, (double*)c->buf[c->cur], c->plane, c->pitch, c->nx, c->ny,
+                                           c->stream)
+                       : launch_ingest_f32(c->staging, (float*)c->buf[c->cur], c->plane, c->pitch, c->nx, c->ny,
+                                           c->stream);
+    HIP_TRY(e);
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->part_valid = false;
+    return 0;
+}
+
+int amhd_download_state(void* ctx, double* U, int on_device) {
+    if (!ctx || !U) return fail(-1, "amhd_download_state: null argument");
+    Ctx* c = (Ctx*)ctx;
+    hipError_t e = c->dtype == AMHD_F64
+                       ? launch_materialize_f64((const double*)c->buf[c->cur], c->plane, c->pitch, c->nx, c->ny,
+                                                c->p.bc_x_periodic, c->p.bc_y_periodic, c->staging, c->stream)
+                       : launch_materialize_f32((const float*)c->buf[c->cur], c->plane, c->pitch, c->nx, c->ny,
+                                                c->p.bc_x_periodic, c->p.bc_y_periodic, c->staging, c->stream);
+    HIP_TRY(e);
+    const size_t gbytes = sizeof(double) * NV * (size_t)(c->nx + 4) * (c->ny + 4);
+    HIP_TRY(hipMemcpyAsync(U, c->staging, gbytes, on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                           c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int amhd_compute_dt(void* ctx, double cfl, double* dt, double* ch) {
+    if (!ctx || !dt || !ch) return fail(-1, "amhd_compute_dt: null argument");
+    Ctx* c = (Ctx*)ctx;
+    int rc = ensure_partials(c);
+    if (rc) return rc;
+    HIP_TRY(enqueue_ctl(c, c->n_part, 0.0, cfl, CTL_DT_ONLY, c->ctl));
+    StepCtl h;
+    HIP_TRY(hipMemcpyAsync(&h, c->ctl, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    *dt = h.dt;
+    *ch = h.ch;
+    return 0;
+}
+
+int amhd_step_ssprk3(void* ctx, double dt, double ch, double psi_decay) {
+    if (!ctx) return fail(-1, "amhd_step_ssprk3: null context");
+    Ctx* c = (Ctx*)ctx;
+    HIP_TRY(enqueue_step(c, c->cur, dt, ch, psi_decay, nullptr));
+    HIP_TRY(enqueue_ctl(c, 0, dt, 0.0, CTL_HOST_STEP_END, c->ctl));
+    c->cur = (c->cur + 1) % 3;
+    c->n_part = n_waves(c);
+    c->part_valid = true;
+    return 0;
+}
+
+int amhd_run(void* ctx, int n_steps, double cfl, double t_end, int max_steps, int use_graph, double* t_inout,
+             int* step_inout, double* last_dt) {
+    if (!ctx || !t_inout || !step_inout) return fail(-1, "amhd_run: null argument");
+    if (n_steps < 0) return fail(-1, "amhd_run: n_steps < 0");
+    Ctx* c = (Ctx*)ctx;
+    StepCtl h;
+    HIP_TRY(hipMemcpyAsync(&h, c->ctl, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    const int start = *step_inout;
+    h.t = *t_inout;
+    h.steps = start;
+    h.t_end = t_end;
+    h.cfl = cfl;
+    h.max_steps = (int)std::min<long>((long)max_steps, (long)start + n_steps);
+    h.glm_alpha = c->p.glm_alpha;
+    h.min_dxdy = std::min(c->p.dx, c->p.dy);
+    h.done = 0;
+    h.pending = 0;
+    HIP_TRY(hipMemcpyAsync(c->ctl, &h, sizeof h, hipMemcpyHostToDevice, c->stream));
+    int rc = ensure_partials(c);
+    if (rc) return rc;
+    HIP_TRY(enqueue_ctl(c, c->n_part, 0.0, cfl, CTL_DEVICE_LOOP, c->ctl));
+    const int nw = n_waves(c);
+    if (use_graph && n_steps > 0) {
+        if (c->graph_rows != c->rows) { drop_graphs(c); c->graph_rows = c->rows; }
+        hipGraphExec_t& ge = c->graph[c->cur];
+        if (!ge) {
+            hipGraph_t g;
+            HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            hipError_t e = hipSuccess;
+            for (int s = 0; s < 3 && e == hipSuccess; ++s) {
+                e = enqueue_step(c, (c->cur + s) % 3, 0.0, 0.0, 1.0, c->ctl);
+                if (e == hipSuccess) e = enqueue_ctl(c, nw, 0.0, cfl, CTL_DEVICE_LOOP, c->ctl);
+            }
+            hipError_t e2 = hipStreamEndCapture(c->stream, &g);
+            HIP_TRY(e);
+            HIP_TRY(e2);
+            HIP_TRY(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+            hipGraphDestroy(g);
+        }
+        for (int r = 0; r < (n_steps + 2) / 3; ++r) HIP_TRY(hipGraphLaunch(ge, c->stream));
+    } else {
+        for (int s = 0; s < n_steps; ++s) {
+            HIP_TRY(enqueue_step(c, (c->cur + s) % 3, 0.0, 0.0, 1.0, c->ctl));
+            HIP_TRY(enqueue_ctl(c, nw, 0.0, cfl, CTL_DEVICE_LOOP, c->ctl));
+        }
+    }
+    HIP_TRY(hipMemcpyAsync(&h, c->ctl, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    const int ran = h.steps - start;
+    c->cur = (c->cur + ran) % 3;
+    if (ran > 0) {
+        c->n_part = nw;
+        c->part_valid = true;
+    }
+    *t_inout = h.t;
+    *step_inout = h.steps;
+    if (last_dt) *last_dt = h.dt;
+    return 0;
+}
+
+int amhd_run_profiled(void* ctx, int n_steps, double cfl, double t_end, int max_steps, double* t_inout,
+                      int* step_inout, double* last_dt, double stage_ms[3], double* ctl_ms) {
+    if (!ctx || !t_inout || !step_inout || !stage_ms) return fail(-1, "amhd_run_profiled: null argument");
+    if (n_steps < 0) return fail(-1, "amhd_run_profiled: n_steps < 0");
+    Ctx* c = (Ctx*)ctx;
+    StepCtl h;
+    HIP_TRY(hipMemcpyAsync(&h, c->ctl, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    const int start = *step_inout;
+    h.t = *t_inout; h.steps = start; h.t_end = t_end; h.cfl = cfl;
+    h.max_steps = (int)std::min<long>((long)max_steps, (long)start + n_steps);
+    h.glm_alpha = c->p.glm_alpha; h.min_dxdy = std::min(c->p.dx, c->p.dy);
+    h.done = 0; h.pending = 0;
+    HIP_TRY(hipMemcpyAsync(c->ctl, &h, sizeof h, hipMemcpyHostToDevice, c->stream));
+    int rc = ensure_partials(c);
+    if (rc) return rc;
+    HIP_TRY(enqueue_ctl(c, c->n_part, 0.0, cfl, CTL_DEVICE_LOOP, c->ctl));
+    const int nw = n_waves(c);
+    // two events around each of the 4 launches of a step
+    std::vector<hipEvent_t> ev((size_t)n_steps * 8);
+    for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
+    hipError_t err = hipSuccess;
+    for (int st = 0; st < n_steps && err == hipSuccess; ++st) {
+        const int cur = (c->cur + st) % 3;
+        void* u0 = c->buf[cur];
+        void* u1 = c->buf[(cur + 1) % 3];
+        void* u2 = c->buf[(cur + 2) % 3];
+        hipEvent_t* e = &ev[(size_t)st * 8];
+        if (err == hipSuccess) err = hipEventRecord(e[0], c->stream);
+        if (err == hipSuccess) err = launch_stage_any(c, 1, u0, u0, u1, 0.0, 0.0, 1.0, c->ctl);
+        if (err == hipSuccess) err = hipEventRecord(e[1], c->stream);
+        if (err == hipSuccess) err = hipEventRecord(e[2], c->stream);
+        if (err == hipSuccess) err = launch_stage_any(c, 2, u0, u1, u2, 0.0, 0.0, 1.0, c->ctl);
+        if (err == hipSuccess) err = hipEventRecord(e[3], c->stream);
+        if (err == hipSuccess) err = hipEventRecord(e[4], c->stream);
+        if (err == hipSuccess) err = launch_stage_any(c, 3, u0, u2, u1, 0.0, 0.0, 1.0, c->ctl);
+        if (err == hipSuccess) err = hipEventRecord(e[5], c->stream);
+        if (err == hipSuccess) err = hipEventRecord(e[6], c->stream);
+        if (err == hipSuccess) err = enqueue_ctl(c, nw, 0.0, cfl, CTL_DEVICE_LOOP, c->ctl);
+        if (err == hipSuccess) err = hipEventRecord(e[7], c->stream);
+    }
+    if (err == hipSuccess) err = hipMemcpyAsync(&h, c->ctl, sizeof h, hipMemcpyDeviceToHost, c->stream);
+    if (err == hipSuccess) err = hipStreamSynchronize(c->stream);
+    double acc[4] = {0, 0, 0, 0};
+    for (int st = 0; st < n_steps && err == hipSuccess; ++st) {
+        for (int k = 0; k < 4 && err == hipSuccess; ++k) {
+            float ms = 0.f;
+            err = hipEventElapsedTime(&ms, ev[(size_t)st * 8 + 2 * k], ev[(size_t)st * 8 + 2 * k + 1]);
+            acc[k] += ms;
+        }
+    }
+    for (auto& e : ev) hipEventDestroy(e);
+    HIP_TRY(err);
+    const int ran = h.steps - start;
+    c->cur = (c->cur + ran) % 3;
+    if (ran > 0) { c->n_part = nw; c->part_valid = true; }
+    *t_inout = h.t;
+    *step_inout = h.steps;
+    if (last_dt) *last_dt = h.dt;
+    for (int k = 0; k < 3; ++k) stage_ms[k] = acc[k];
+    if (ctl_ms) *ctl_ms = acc[3];
+    return 0;
+}
+
+int amhd_boundary_energy(void* ctx, double* cumulative, double* last) {
+    if (!ctx) return fail(-1, "amhd_boundary_energy: null context");
+    Ctx* c = (Ctx*)ctx;
+    StepCtl h;
+    HIP_TRY(hipMemcpyAsync(&h, c->ctl, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (cumulative) *cumulative = h.bflux_cum;
+    if (last) *last = h.bflux;
+    return 0;
+}
+
+int amhd_reset_boundary_energy(void* ctx, double value) {
+    if (!ctx) return fail(-1, "amhd_reset_boundary_energy: null context");
+    Ctx* c = (Ctx*)ctx;
+    StepCtl h;
+    HIP_TRY(hipMemcpyAsync(&h, c->ctl, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    h.bflux_cum = value;
+    h.bflux = 0.0;
+    HIP_TRY(hipMemcpyAsync(c->ctl, &h, sizeof h, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int amhd_compute_rhs(void* ctx, double* U, double* R, double ch) {
+    if (!ctx || !U || !R) return fail(-1, "amhd_compute_rhs: null argument");
+    Ctx* c = (Ctx*)ctx;
+    const int nxt = c->nx + 4, nyt = c->ny + 4;
+    const long gplane = (long)nxt * nyt;
+    apply_bc_kernel<<<grid_for(gplane, 256), 256, 0, c->stream>>>(U, c->nx, c->ny, c->p.bc_x_periodic,
+                                                                   c->p.bc_y_periodic);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemsetAsync(R, 0, sizeof(double) * NV * gplane, c->stream));
+    StageParams<double> a{};
+    a.u0 = a.us = U + 2 * nyt + 2;
+    a.uo = R + 2 * nyt + 2;
+    a.plane = a.oplane = gplane;
+    a.pitch = a.opitch = nyt;
+    a.nx = c->nx; a.ny = c->ny;
+    a.bcx_periodic = c->p.bc_x_periodic; a.bcy_periodic = c->p.bc_y_periodic;
+    a.n_strips = c->n_strips; a.n_segs = c->n_segs; a.rows = c->rows;
+    a.gamma = c->p.gamma; a.dx = c->p.dx; a.dy = c->p.dy;
+    a.dt = 0.0; a.ch = ch; a.decay = 1.0;
+    HIP_TRY(launch_stage_f64(0, c->p.powell != 0, a, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int amhd_global_sums(void* ctx, double out[4]) {
+    if (!ctx || !out) return fail(-1, "amhd_global_sums: null argument");
+    Ctx* c = (Ctx*)ctx;
+    const int g = grid_for((long)c->nx * c->ny, 256, GSUM_BLOCKS);
+    if (c->dtype == AMHD_F64)
+        global_sums_kernel<double><<<g, 256, 0, c->stream>>>((const double*)c->buf[c->cur], c->plane, c->pitch, c->nx,
+                                                             c->ny, c->p.dx, c->p.dy, c->gsum);
+    else
+        global_sums_kernel<float><<<g, 256, 0, c->stream>>>((const float*)c->buf[c->cur], c->plane, c->pitch, c->nx,
+                                                            c->ny, c->p.dx, c->p.dy, c->gsum);
+    HIP_TRY(hipGetLastError());
+    double h[4 * GSUM_BLOCKS];
+    HIP_TRY(hipMemcpyAsync(h, c->gsum, sizeof(double) * 4 * g, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    double a = 0, b = 0, m = 0, d2 = 0;
+    for (int k = 0; k < g; ++k) { a += h[4 * k]; b += h[4 * k + 1]; m = std::max(m, h[4 * k + 2]); d2 += h[4 * k + 3]; }
+    out[0] = a * c->p.dx * c->p.dy;
+    out[1] = b * c->p.dx * c->p.dy;
+    out[2] = m;
+    out[3] = std::sqrt(d2 / ((double)c->nx * c->ny));
+    return 0;
+}
+
+// ------------------------------------------------------------- standalone
+int amhd_cons_to_prim(const double* U, double* W, long n, double gamma, void* stream) {
+    if (!U || !W || n < 0) return fail(-1, "amhd_cons_to_prim: bad argument");
+    if (n == 0) return 0;
+    cons_to_prim_kernel<<<grid_for(n, 256), 256, 0, (hipStream_t)stream>>>(U, W, n, gamma);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int amhd_prim_to_cons(const double* W, double* U, long n, double gamma, void* stream) {
+    if (!U || !W || n < 0) return fail(-1, "amhd_prim_to_cons: bad argument");
+    if (n == 0) return 0;
+    prim_to_cons_kernel<<<grid_for(n, 256), 256, 0, (hipStream_t)stream>>>(W, U, n, gamma);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int amhd_muscl(const double* W, int nxt, int nyt, int dir, double* WL, double* WR, void* stream) {
+    if (!W || !WL || !WR) return fail(-1, "amhd_muscl: null argument");
+    if (dir != 0 && dir != 1) return fail(-1, "amhd_muscl: dir must be 0 or 1");
+    if ((dir == 0 && nxt < 4) || (dir == 1 && nyt < 4) || nxt < 1 || nyt < 1)
+        return fail(-1, "amhd_muscl: need at least 4 cells along the reconstruction axis");
+    const long nf = dir == 0 ? (long)(nxt - 3) * nyt : (long)nxt * (nyt - 3);
+    muscl_kernel<<<grid_for(nf, 256), 256, 0, (hipStream_t)stream>>>(W, nxt, nyt, dir, WL, WR);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int amhd_hlld_flux(const void* WL, const void* WR, void* F, double* smax, long n, int dir, double gamma, double ch,
+                   double floor_rho, double floor_pr, int dtype, void* stream) {
+    if (!WL || !WR || !F || n < 0) return fail(-1, "amhd_hlld_flux: bad argument");
+    if (dir != 0 && dir != 1) return fail(-1, "amhd_hlld_flux: dir must be 0 or 1");
+    hipStream_t s = (hipStream_t)stream;
+    unsigned long long* bits = nullptr;
+    if (smax) {
+        Scratch* sc;
+        int rc = get_scratch(&sc);
+        if (rc) return rc;
+        bits = sc->bits;
+        HIP_TRY(hipMemsetAsync(bits, 0, sizeof *bits, s));
+    }
+    if (n > 0) {
+        if (dtype == AMHD_F64) {
+            const FluxConst<double> kc = make_flux_const<double>(gamma, ch, floor_rho, floor_pr);
+            hlld_kernel<double, ExactMath<double>><<<grid_for(n, 256), 256, 0, s>>>(
+                (const double*)WL, (const double*)WR, (double*)F, n, dir, kc, bits);
+            HIP_TRY(hipGetLastError());
+        } else if (dtype == AMHD_F32) {
+            HIP_TRY(launch_hlld_f32((const float*)WL, (const float*)WR, (float*)F, n, dir, gamma, ch, floor_rho,
+                                    floor_pr, bits, s));
+        } else {
+            return fail(-1, "amhd_hlld_flux: unknown dtype %d", dtype);
+        }
+    }
+    if (smax) {
+        unsigned long long h = 0;
+        HIP_TRY(hipMemcpyAsync(&h, bits, sizeof h, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        double v;
+        std::memcpy(&v, &h, sizeof v);
+        *smax = v;
+    }
+    return 0;
+}
+
+int amhd_apply_bc(double* U, int nx, int ny, int bcx, int bcy, void* stream) {
+    if (!U || nx < 2 || ny < 2) return fail(-1, "amhd_apply_bc: bad argument");
+    const long gplane = (long)(nx + 4) * (ny + 4);
+    apply_bc_kernel<<<grid_for(gplane, 256), 256, 0, (hipStream_t)stream>>>(U, nx, ny, bcx, bcy);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int amhd_enforce_scalar_bounds(double* U, long n, void* stream) {
+    if (!U || n < 0) return fail(-1, "amhd_enforce_scalar_bounds: bad argument");
+    if (n == 0) return 0;
+    enforce_bounds_kernel<<<grid_for(n, 256), 256, 0, (hipStream_t)stream>>>(U, n);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int amhd_compute_dt_w(const double* W, long n, double gamma, double cfl, double dx, double dy, double* dt,
+                      double* ch, void* stream) {
+    if (!W || n <= 0 || !dt || !ch) return fail(-1, "amhd_compute_dt_w: bad argument");
+    Scratch* sc;
+    int rc = get_scratch(&sc);
+    if (rc) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    const int g = grid_for(n, 256, SCRATCH_PARTS);
+    dt_partials_w_kernel<<<g, 256, 0, s>>>(W, n, gamma, sc->part);
+    HIP_TRY(hipGetLastError());
+    step_ctl_kernel<<<1, 256, 0, s>>>(sc->ctl, sc->part, g, nullptr, 0, dx, dy, 0.0, cfl, CTL_DT_ONLY);
+    HIP_TRY(hipGetLastError());
+    StepCtl h;
+    HIP_TRY(hipMemcpyAsync(&h, sc->ctl, sizeof h, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    *dt = h.dt;
+    *ch = h.ch;
+    return 0;
+}
+
+}  // extern "C"

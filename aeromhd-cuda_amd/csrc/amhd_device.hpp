@@ -1,0 +1,361 @@
+// amhd_device.hpp -- device-side building blocks of the MHD update on gfx950.
+//
+// Everything here is templated on the arithmetic type `real` and a math
+// policy `P`:
+//   * ExactMath<double>: the reference's IEEE arithmetic (true division,
+//     correctly rounded sqrt, reference epsilons).  Compiled with
+//     -ffp-contract=off this reproduces the reference's FP64 per-operation
+//     results bit for bit.
+//   * FastMath<float>: FP32 with hardware reciprocal/sqrt (v_rcp_f32,
+//     v_sqrt_f32, ~1 ulp), hoisted common reciprocals and epsilons scaled to
+//     FP32 resolution.  Its error is measured against the FP64 oracle.
+//
+// Variable order (config.py:13-22):
+//   conserved [rho, mx, my, mz, Bx, By, Bz, E, psi, rhoC]
+//   primitive [rho, vx, vy, vz, Bx, By, Bz, p, psi, C]
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace amhd {
+
+constexpr int NV = 10;
+enum : int { RHO = 0, MX, MY, MZ, BX, BY, BZ, EN, PSI, RHOC };
+enum : int { iRHO = 0, iVX, iVY, iVZ, iBX, iBY, iBZ, iPR, iPSI, iCLR };
+constexpr double FLOOR_RHO = 1e-12;  // config.py:40
+constexpr double FLOOR_PR = 1e-12;   // config.py:41
+
+// ------------------------------------------------------------------ policies
+template <class real> struct ExactMath {
+    static constexpr bool exact = true;
+    static __device__ __forceinline__ real rcp(real b) { return real(1) / b; }
+    // a / b; `ib` (a precomputed reciprocal) is ignored so the quotient is the
+    // correctly rounded IEEE one.
+    static __device__ __forceinline__ real divi(real a, real b, real /*ib*/) { return a / b; }
+    static __device__ __forceinline__ real div(real a, real b) { return a / b; }
+    static __device__ __forceinline__ real sqrt_(real a) { return sqrt(a); }
+    // epsilons of physics.py:114,117-118,122-123,130
+    static __device__ __forceinline__ real eps_den() { return real(1e-14); }
+    static __device__ __forceinline__ real eps_sm(real /*sl*/, real /*sr*/) { return real(1e-14); }
+    static __device__ __forceinline__ real eps_rel_scale() { return real(1e-8); }
+};
+
+template <class real> struct FastMath;
+template <> struct FastMath<float> {
+    static constexpr bool exact = false;
+    static __device__ __forceinline__ float rcp(float b) { return __builtin_amdgcn_rcpf(b); }
+    static __device__ __forceinline__ float divi(float a, float /*b*/, float ib) { return a * ib; }
+    static __device__ __forceinline__ float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+    static __device__ __forceinline__ float sqrt_(float a) { return __builtin_amdgcn_sqrtf(a); }
+    // FP32 cannot resolve the reference's absolute 1e-14 next to O(1..10)
+    // speeds: SL + 1e-14 == SL.  The SM clamp therefore uses a margin of
+    // 2^-20 of the fan width, and the degenerate-denominator guard a
+    // relative 1e-5 instead of 1e-8.
+    static __device__ __forceinline__ float eps_den() { return 1e-14f; }
+    static __device__ __forceinline__ float eps_sm(float sl, float sr) {
+        return fmaxf(1e-14f, 9.5367431640625e-07f * (sr - sl));
+    }
+    static __device__ __forceinline__ float eps_rel_scale() { return 1e-5f; }
+};
+
+// ------------------------------------------------------------- wave shifts
+// Whole-wavefront DPP shifts (gfx9 wave_shr:1 / wave_shl:1).  Lane l of
+// from_prev() receives lane l-1's value; lane l of from_next() receives lane
+// l+1's.  Lane 0 / lane 63 receive 0 (they are halo lanes and never output).
+__device__ __forceinline__ float from_prev(float x) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0x138, 0xf, 0xf, true));
+}
+__device__ __forceinline__ float from_next(float x) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0x130, 0xf, 0xf, true));
+}
+__device__ __forceinline__ double from_prev(double x) {
+    int lo = __builtin_amdgcn_mov_dpp(__double2loint(x), 0x138, 0xf, 0xf, true);
+    int hi = __builtin_amdgcn_mov_dpp(__double2hiint(x), 0x138, 0xf, 0xf, true);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double from_next(double x) {
+    int lo = __builtin_amdgcn_mov_dpp(__double2loint(x), 0x130, 0xf, 0xf, true);
+    int hi = __builtin_amdgcn_mov_dpp(__double2hiint(x), 0x130, 0xf, 0xf, true);
+    return __hiloint2double(hi, lo);
+}
+
+// Max over the 64 lanes of a wave (result valid in every lane).
+__device__ __forceinline__ double wave_max(double v) {
+    for (int off = 32; off > 0; off >>= 1) {
+        double o = __shfl_xor(v, off, 64);
+        v = o > v ? o : v;
+    }
+    return v;
+}
+__device__ __forceinline__ double wave_sum(double v) {
+    // Fixed butterfly order: deterministic for a given lane assignment.
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// ------------------------------------------------------------- index remap
+// Ghost handling of solver.py:158-197 as an index map onto interior cells:
+// periodic -> wrap, otherwise zero-gradient copy of the nearest interior
+// cell.  Valid for any integer i (the y halo lanes may run far past ny).
+__device__ __forceinline__ int remap(int i, int n, int periodic) {
+    if (periodic) {
+        int r = i % n;
+        return r < 0 ? r + n : r;
+    }
+    return i < 0 ? 0 : (i >= n ? n - 1 : i);
+}
+
+// --------------------------------------------------------------- primitives
+// physics.py:284-301 for one cell.  `psi_zero`: x-ghost of a non-periodic x
+// boundary, where solver.py:174-176 sets psi = 0.
+template <class real, class P>
+__device__ __forceinline__ void cons_to_prim(const real (&u)[NV], real (&w)[NV], real gamma, bool psi_zero) {
+    real rho = u[RHO] > real(FLOOR_RHO) ? u[RHO] : real(FLOOR_RHO);
+    const real ri = P::rcp(rho);
+    real vx = u[MX] * ri, vy = u[MY] * ri, vz = u[MZ] * ri;
+    real bx = u[BX], by = u[BY], bz = u[BZ];
+    w[iRHO] = rho; w[iVX] = vx; w[iVY] = vy; w[iVZ] = vz;
+    w[iBX] = bx; w[iBY] = by; w[iBZ] = bz;
+    w[iPSI] = psi_zero ? real(0) : u[PSI];
+    w[iCLR] = u[RHOC] * ri;
+    real ke = real(0.5) * rho * (vx * vx + vy * vy + vz * vz);
+    real me = real(0.5) * (bx * bx + by * by + bz * bz);
+    real p = (gamma - real(1)) * (u[EN] - ke - me);
+    w[iPR] = p > real(FLOOR_PR) ? p : real(FLOOR_PR);
+}
+
+// van Leer limited slope, physics.py:325-328.
+template <class real, class P>
+__device__ __forceinline__ real vl_slope(real wm, real w0, real wp) {
+    real dm = w0 - wm, dp = wp - w0;
+    real ab = dm * dp;
+    real d = P::div(real(2) * ab, dm + dp + real(1e-30));
+    return ab > real(0) ? d : real(0);
+}
+
+// ---------------------------------------------------------------- HLLD face
+// Constants of one flux evaluation; derived once per launch so that every
+// per-face quotient the reference evaluates (0.5/ch, ch*ch) has the same
+// IEEE value.
+template <class real> struct FluxConst {
+    real gamma, gm1, igm1;  // gamma, gamma-1, 1/(gamma-1) (fast path only)
+    real ch, half_over_ch, half_ch, ch2;
+    real floor_rho, floor_pr;
+};
+
+template <class real>
+__host__ __device__ inline FluxConst<real> make_flux_const(double gamma, double ch, double floor_rho = FLOOR_RHO,
+                                                           double floor_pr = FLOOR_PR) {
+    FluxConst<real> c;
+    c.floor_rho = real(floor_rho);
+    c.floor_pr = real(floor_pr);
+    c.gamma = real(gamma);
+    c.gm1 = real(gamma - 1.0);
+    c.igm1 = real(1.0 / (gamma - 1.0));
+    c.ch = real(ch);
+    c.half_over_ch = ch > 0.0 ? real(0.5 / ch) : real(0);
+    c.half_ch = real(0.5 * ch);
+    c.ch2 = real(ch * ch);
+    return c;
+}
+
+// The HLLD x-flux of physics.py:43-275 for one face.  wl, wr are primitive
+// vectors (for y faces the caller passes them with vx<->vy, Bx<->By swapped
+// and swaps f back, physics.py:373-379).  The reference evaluates all four
+// intermediate fluxes and picks one; here the side K of the contact that
+// holds the face (K = L iff SM >= 0, since the clamps keep SAL <= SM <= SAR)
+// is selected first and only F_K, F*_K and F**_K are formed.  Each value is
+// computed with the reference's operation order, so the FP64 result is
+// identical.  Returns max(|SL|, |SR|).
+template <class real, class P>
+__device__ __forceinline__ real hlld_face(const real (&wl)[NV], const real (&wr)[NV], real (&f)[NV],
+                                          const FluxConst<real>& k) {
+    const real half = real(0.5);
+    real rhoL = wl[0] < k.floor_rho ? k.floor_rho : wl[0];
+    real rhoR = wr[0] < k.floor_rho ? k.floor_rho : wr[0];
+    real vxL = wl[1], vyL = wl[2], vzL = wl[3];
+    real vxR = wr[1], vyR = wr[2], vzR = wr[3];
+    real bxL = wl[4], byL = wl[5], bzL = wl[6];
+    real bxR = wr[4], byR = wr[5], bzR = wr[6];
+    real pL = wl[7] < k.floor_pr ? k.floor_pr : wl[7];
+    real pR = wr[7] < k.floor_pr ? k.floor_pr : wr[7];
+    real psiL = wl[8], psiR = wr[8];
+    real cL = wl[9], cR = wr[9];
+    const bool glm = k.ch > real(0);
+
+    // GLM face Bx / psi (physics.py:74-82)
+    real bxf, psif;
+    if (glm) {
+        bxf = half * (bxL + bxR) - k.half_over_ch * (psiR - psiL);
+        psif = half * (psiL + psiR) - k.half_ch * (bxR - bxL);
+    } else {
+        bxf = half * (bxL + bxR);
+        psif = real(0);
+    }
+    const real bx2 = bxf * bxf;
+    const real bmL2 = bx2 + byL * byL + bzL * bzL;
+    const real bmR2 = bx2 + byR * byR + bzR * bzR;
+    const real ptL = pL + half * bmL2;
+    const real ptR = pR + half * bmR2;
+
+    // fast magnetosonic speeds (physics.py:90-102)
+    const real irL = P::rcp(rhoL), irR = P::rcp(rhoR);
+    real aL2 = P::divi(k.gamma * pL, rhoL, irL);
+    real aR2 = P::divi(k.gamma * pR, rhoR, irR);
+    real tmpL = aL2 + P::divi(bmL2, rhoL, irL);
+    real tmpR = aR2 + P::divi(bmR2, rhoR, irR);
+    real discL = tmpL * tmpL - P::divi(real(4) * aL2 * bx2, rhoL, irL);
+    real discR = tmpR * tmpR - P::divi(real(4) * aR2 * bx2, rhoR, irR);
+    if (discL < real(0)) discL = real(0);
+    if (discR < real(0)) discR = real(0);
+    const real cfL = P::sqrt_(fmax(half * (tmpL + P::sqrt_(discL)), real(0)));
+    const real cfR = P::sqrt_(fmax(half * (tmpR + P::sqrt_(discR)), real(0)));
+
+    real sl = fmin(vxL - cfL, vxR - cfR);
+    real sr = fmax(vxL + cfL, vxR + cfR);
+    if (glm) { sl = fmin(sl, -k.ch); sr = fmax(sr, k.ch); }
+    const real smax = fmax(fabs(sl), fabs(sr));
+
+    // contact speed (physics.py:111-120)
+    const real dsL = sl - vxL, dsR = sr - vxR;
+    real den = dsR * rhoR - dsL * rhoL;
+    if (fabs(den) < P::eps_den()) den = (den >= real(0)) ? P::eps_den() : -P::eps_den();
+    real sm = P::div(dsR * rhoR * vxR - dsL * rhoL * vxL - ptR + ptL, den);
+    {
+        const real e = P::eps_sm(sl, sr);
+        if (sm < sl + e) sm = sl + e;
+        if (sm > sr - e) sm = sr - e;
+    }
+    const real pts = ptL + rhoL * dsL * (sm - vxL);
+
+    // star states (physics.py:122-154)
+    real dslsm = sl - sm; if (fabs(dslsm) < P::eps_den()) dslsm = P::eps_den();
+    real dsrsm = sr - sm; if (fabs(dsrsm) < P::eps_den()) dsrsm = P::eps_den();
+    const real idslsm = P::rcp(dslsm), idsrsm = P::rcp(dsrsm);
+    const real rhoLS = fmax(P::divi(rhoL * dsL, dslsm, idslsm), k.floor_rho);
+    const real rhoRS = fmax(P::divi(rhoR * dsR, dsrsm, idsrsm), k.floor_rho);
+    const real sqLS = P::sqrt_(rhoLS);
+    const real sqRS = P::sqrt_(rhoRS);
+
+    const real eps_rel = P::eps_rel_scale() * fmax(fabs(pts), real(1));
+    real dnL = rhoL * dsL * dslsm - bx2;
+    real dnR = rhoR * dsR * dsrsm - bx2;
+    if (fabs(dnL) < eps_rel) dnL = (dnL >= real(0)) ? eps_rel : -eps_rel;
+    if (fabs(dnR) < eps_rel) dnR = (dnR >= real(0)) ? eps_rel : -eps_rel;
+    const real idnL = P::rcp(dnL), idnR = P::rcp(dnR);
+
+    const real vyLS = vyL - P::divi(bxf * byL * (sm - vxL), dnL, idnL);
+    const real vzLS = vzL - P::divi(bxf * bzL * (sm - vxL), dnL, idnL);
+    const real byLS = P::divi(byL * (rhoL * dsL * dsL - bx2), dnL, idnL);
+    const real bzLS = P::divi(bzL * (rhoL * dsL * dsL - bx2), dnL, idnL);
+    const real vyRS = vyR - P::divi(bxf * byR * (sm - vxR), dnR, idnR);
+    const real vzRS = vzR - P::divi(bxf * bzR * (sm - vxR), dnR, idnR);
+    const real byRS = P::divi(byR * (rhoR * dsR * dsR - bx2), dnR, idnR);
+    const real bzRS = P::divi(bzR * (rhoR * dsR * dsR - bx2), dnR, idnR);
+
+    const real vbL = vxL * bxf + vyL * byL + vzL * bzL;
+    const real vbLS = sm * bxf + vyLS * byLS + vzLS * bzLS;
+    const real eL = P::divi(pL, k.gm1, k.igm1) + half * rhoL * (vxL * vxL + vyL * vyL + vzL * vzL) + half * bmL2;
+    const real eLS = P::divi((sl - vxL) * eL - ptL * vxL + pts * sm + bxf * (vbL - vbLS), dslsm, idslsm);
+    const real vbR = vxR * bxf + vyR * byR + vzR * bzR;
+    const real vbRS = sm * bxf + vyRS * byRS + vzRS * bzRS;
+    const real eR = P::divi(pR, k.gm1, k.igm1) + half * rhoR * (vxR * vxR + vyR * vyR + vzR * vzR) + half * bmR2;
+    const real eRS = P::divi((sr - vxR) * eR - ptR * vxR + pts * sm + bxf * (vbR - vbRS), dsrsm, idsrsm);
+
+    // double-star state (physics.py:156-172)
+    const real sbx = (bxf >= real(0)) ? real(1) : real(-1);
+    const real abx = fabs(bxf);
+    real sal = sm - P::div(abx, fmax(sqLS, real(1e-14)));
+    real sar = sm + P::div(abx, fmax(sqRS, real(1e-14)));
+    if (sal < sl) sal = sl;
+    if (sal > sm) sal = sm;
+    if (sar < sm) sar = sm;
+    if (sar > sr) sar = sr;
+    const real dds = fmax(sqLS + sqRS, real(1e-14));
+    const real idds = P::rcp(dds);
+    const real vyDS = P::divi(sqLS * vyLS + sqRS * vyRS + (byRS - byLS) * sbx, dds, idds);
+    const real vzDS = P::divi(sqLS * vzLS + sqRS * vzRS + (bzRS - bzLS) * sbx, dds, idds);
+    const real byDS = P::divi(sqLS * byRS + sqRS * byLS + sqLS * sqRS * (vyRS - vyLS) * sbx, dds, idds);
+    const real bzDS = P::divi(sqLS * bzRS + sqRS * bzLS + sqLS * sqRS * (vzRS - vzLS) * sbx, dds, idds);
+    const real vbDS = sm * bxf + vyDS * byDS + vzDS * bzDS;
+
+    // region: supersonic L / R, or the star fan split by the contact
+    const bool supL = sl >= real(0);
+    const bool supR = !supL && sr <= real(0);
+    const bool star = !supL && !supR;
+    const bool useL = supL || (star && sm >= real(0));
+    const bool dbl = star && (useL ? (sal < real(0)) : (sar >= real(0)));
+
+    // side-K quantities
+    const real rho = useL ? rhoL : rhoR;
+    const real vx = useL ? vxL : vxR;
+    const real vy = useL ? vyL : vyR;
+    const real vz = useL ? vzL : vzR;
+    const real by = useL ? byL : byR;
+    const real bz = useL ? bzL : bzR;
+    const real pt = useL ? ptL : ptR;
+    const real e = useL ? eL : eR;
+    const real vb = useL ? vbL : vbR;
+    const real psi = useL ? psiL : psiR;
+    const real c = useL ? cL : cR;
+    const real sk = useL ? sl : sr;
+    const real sak = useL ? sal : sar;
+    const real rhos = useL ? rhoLS : rhoRS;
+    const real vys = useL ? vyLS : vyRS;
+    const real vzs = useL ? vzLS : vzRS;
+    const real bys = useL ? byLS : byRS;
+    const real bzs = useL ? bzLS : bzRS;
+    const real es = useL ? eLS : eRS;
+    // E** (physics.py:171-172): L side subtracts, R side adds
+    const real eds = useL ? (eLS - sqLS * (vbLS - vbDS) * sbx) : (eRS + sqRS * (vbRS - vbDS) * sbx);
+
+    // physical flux of side K (physics.py:178-187)
+    real fk[NV];
+    fk[0] = rho * vx;
+    fk[1] = rho * vx * vx + pt - bx2;
+    fk[2] = rho * vy * vx - bxf * by;
+    fk[3] = rho * vz * vx - bxf * bz;
+    fk[4] = glm ? psif : real(0);
+    fk[5] = by * vx - bxf * vy;
+    fk[6] = bz * vx - bxf * vz;
+    fk[7] = (e + pt) * vx - bxf * vb;
+    fk[8] = glm ? bxf * k.ch * k.ch : real(0);
+    fk[9] = rho * c * vx;
+    // U_K, U*_K, U**_K (physics.py:213-249)
+    const real uk[NV] = {rho, rho * vx, rho * vy, rho * vz, bxf, by, bz, e, psi, rho * c};
+    const real us[NV] = {rhos, rhos * sm, rhos * vys, rhos * vzs, bxf, bys, bzs, es, psif, rhos * c};
+    const real uds[NV] = {rhos, rhos * sm, rhos * vyDS, rhos * vzDS, bxf, byDS, bzDS, eds, psif, rhos * c};
+
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        real fs = fk[v] + sk * (us[v] - uk[v]);
+        real fds;
+        if constexpr (P::exact) {
+            fds = fs + sak * (uds[v] - us[v]);
+        } else {
+            // components 0, 1, 4, 8, 9 of U** equal those of U*
+            fds = (v == 0 || v == 1 || v == 4 || v == 8 || v == 9) ? fs : fs + sak * (uds[v] - us[v]);
+        }
+        f[v] = star ? (dbl ? fds : fs) : fk[v];
+    }
+    if (glm) {  // physics.py:265-268 (every glm face is in the star fan)
+        f[4] = psif;
+        f[8] = k.ch2 * bxf;
+    }
+    return smax;
+}
+
+// y-face flux: rotate (vx,vy),(Bx,By), solve, rotate back (physics.py:373-379)
+template <class real, class P>
+__device__ __forceinline__ real hlld_face_y(const real (&wl)[NV], const real (&wr)[NV], real (&f)[NV],
+                                            const FluxConst<real>& k) {
+    real a[NV] = {wl[0], wl[2], wl[1], wl[3], wl[5], wl[4], wl[6], wl[7], wl[8], wl[9]};
+    real b[NV] = {wr[0], wr[2], wr[1], wr[3], wr[5], wr[4], wr[6], wr[7], wr[8], wr[9]};
+    real g[NV];
+    real s = hlld_face<real, P>(a, b, g, k);
+    f[0] = g[0]; f[1] = g[2]; f[2] = g[1]; f[3] = g[3]; f[4] = g[5];
+    f[5] = g[4]; f[6] = g[6]; f[7] = g[7]; f[8] = g[8]; f[9] = g[9];
+    return s;
+}
+
+}  // namespace amhd

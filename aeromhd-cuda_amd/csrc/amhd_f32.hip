@@ -1,0 +1,34 @@
+// amhd_f32.hip -- FP32 instantiation of the fused update (FastMath policy).
+// Built with -ffp-contract=fast: FP32 results are not bit-matched to the
+// reference; their error is measured against the FP64 oracle.
+#define AMHD_LAUNCH_IMPL
+#include "amhd_launch.hpp"
+
+namespace amhd {
+
+hipError_t launch_stage_f32(int stage, bool powell, const StageParams<float>& a, hipStream_t s) {
+    return launch_stage_impl<float, FastMath<float>>(stage, powell, a, s);
+}
+
+int launch_dt_partials_f32(const float* u, long plane, int pitch, int nx, int ny, double gamma, double* part,
+                           int max_parts, hipStream_t s) {
+    return launch_dt_partials_impl<float>(u, plane, pitch, nx, ny, gamma, part, max_parts, s);
+}
+
+hipError_t launch_ingest_f32(const double* ug, float* u, long plane, int pitch, int nx, int ny, hipStream_t s) {
+    return launch_ingest_impl<float>(ug, u, plane, pitch, nx, ny, s);
+}
+
+hipError_t launch_materialize_f32(const float* u, long plane, int pitch, int nx, int ny, int bcx, int bcy,
+                                  double* ug, hipStream_t s) {
+    return launch_materialize_impl<float>(u, plane, pitch, nx, ny, bcx, bcy, ug, s);
+}
+
+hipError_t launch_hlld_f32(const float* wl, const float* wr, float* f, long n, int dir, double gamma, double ch,
+                           double floor_rho, double floor_pr, unsigned long long* smax_bits, hipStream_t s) {
+    const FluxConst<float> kc = make_flux_const<float>(gamma, ch, floor_rho, floor_pr);
+    hlld_kernel<float, FastMath<float>><<<grid_for(n, 256), 256, 0, s>>>(wl, wr, f, n, dir, kc, smax_bits);
+    return hipGetLastError();
+}
+
+}  // namespace amhd

@@ -1,0 +1,616 @@
+// amhd_kernels.hpp -- gfx950 kernels of the MHD update.
+//
+// State layout in HBM ("interior layout"): NV planes, each nx rows of ny
+// cells (x slow, y fast), row pitch `pitch` elements, plane stride `plane`
+// elements.  Ghost cells are not stored: every read of a cell outside the
+// interior goes through remap() (periodic wrap or zero-gradient clamp, and
+// psi = 0 on the x-ghosts of a non-periodic x boundary), which is what
+// solver.py:158-197 leaves in the ghost cells before every RHS evaluation.
+//
+// The fused stage kernel (one launch per SSP-RK3 stage) is a strip march:
+// each wavefront owns a strip of STRIP = 60 consecutive y cells (64 lanes:
+// 2 halo lanes on each side) and walks down a segment of `rows` x rows.
+// Per row it loads one new conserved row, converts it to primitives once,
+// and keeps a 3-row primitive window in registers: the x-face MUSCL + HLLD
+// uses the window, the y-face MUSCL + HLLD uses DPP lane shifts, the x-flux
+// of the previous row is carried in registers, so each face is solved once
+// (apart from one extra x-face per segment and the 4 halo lanes).  The
+// epilogue applies the Powell source, the RK stage combine, the scalar
+// bounds, the psi decay (stage 3) and writes per-wave partial maxima for the
+// next CFL step.
+#pragma once
+#include "amhd_device.hpp"
+
+namespace amhd {
+
+constexpr int WAVE = 64;
+constexpr int HALO = 2;
+constexpr int STRIP = WAVE - 2 * HALO;  // output columns per wavefront
+constexpr int WPB = 4;                  // wavefronts per workgroup
+constexpr int NPART_DT = 4;             // {max cf, max(|vx|+|vy|), max|vx|, max|vy|}
+
+// Device-resident step control for the host-free time loop.
+struct StepCtl {
+    double t, dt, ch, decay;
+    double t_end, cfl, glm_alpha, min_dxdy;
+    double bflux, bflux_cum;
+    double cfmax, vmax;
+    int steps, max_steps, done, pending;
+};
+
+template <class real> struct StageParams {
+    const real* u0;   // U^n (interior layout)
+    const real* us;   // stage input U^(s)
+    real* uo;         // stage output (interior layout), or R (STAGE 0)
+    long plane;       // plane stride of u0/us/uo
+    long oplane;      // plane stride of uo (R output may be ghosted)
+    int pitch, opitch;
+    int nx, ny, bcx_periodic, bcy_periodic;
+    int n_strips, n_segs, rows;
+    double gamma, dx, dy;
+    double dt, ch, decay;   // used when ctl == nullptr
+    const StepCtl* ctl;     // device-resident dt/ch/decay
+    double* part_dt;        // STAGE 3: [n_waves][4]
+    double* part_bf;        // STAGE 1: [n_strips][2] (non-periodic x only)
+};
+
+// XCD-aware workgroup order: the hardware deals workgroups round-robin over
+// the 8 XCDs; renumber so that consecutive logical workgroups (neighbouring
+// strips, which share halo columns) land on the same XCD and its L2.
+__device__ __forceinline__ int xcd_logical_block() {
+    const int b = blockIdx.x, nb = gridDim.x;
+    const int nb8 = nb & ~7;
+    return b < nb8 ? (b & 7) * (nb8 >> 3) + (b >> 3) : b;
+}
+
+template <class real>
+__device__ __forceinline__ void floor_face(real (&w)[NV]) {
+    if (w[iRHO] < real(FLOOR_RHO)) w[iRHO] = real(FLOOR_RHO);
+    if (w[iPR] < real(FLOOR_PR)) w[iPR] = real(FLOOR_PR);
+}
+
+template <class real>
+__device__ __forceinline__ void load_cell(const real* __restrict__ base, long plane, real (&u)[NV]) {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) u[v] = base[v * plane];
+}
+
+// FP64 CFL contributions of one cell (solver.py:209-214), from conserved u.
+template <class real>
+__device__ __forceinline__ void dt_terms(const real (&u)[NV], double gamma, double (&m)[NPART_DT]) {
+    double ud[NV], w[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) ud[v] = double(u[v]);
+    cons_to_prim<double, ExactMath<double>>(ud, w, gamma, false);
+    const double b2 = w[iBX] * w[iBX] + w[iBY] * w[iBY] + w[iBZ] * w[iBZ];
+    const double cf = sqrt(gamma * w[iPR] / w[iRHO] + b2 / w[iRHO]);
+    const double ax = fabs(w[iVX]), ay = fabs(w[iVY]);
+    m[0] = fmax(m[0], cf);
+    m[1] = fmax(m[1], ax + ay);
+    m[2] = fmax(m[2], ax);
+    m[3] = fmax(m[3], ay);
+}
+
+// Boundary energy-flux density of one cell (solver.py:233-239), FP64.
+template <class real>
+__device__ __forceinline__ double bflux_term(const real (&u)[NV], double gamma, double dy) {
+    double ud[NV], w[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) ud[v] = double(u[v]);
+    cons_to_prim<double, ExactMath<double>>(ud, w, gamma, false);
+    const double b2 = w[iBX] * w[iBX] + w[iBY] * w[iBY] + w[iBZ] * w[iBZ];
+    const double pt = w[iPR] + 0.5 * b2;
+    const double vb = w[iVX] * w[iBX] + w[iVY] * w[iBY] + w[iVZ] * w[iBZ];
+    return ((ud[EN] + pt) * w[iVX] - w[iBX] * vb) * dy;
+}
+
+// ------------------------------------------------------------ stage kernel
+// STAGE 0: R = L(U^s) only (compute_rhs, solver.py:268-308)
+// STAGE 1: U1 = U0 + dt L(U0)                              (solver.py:319)
+// STAGE 2: U2 = 0.75 U0 + 0.25 (U1 + dt L(U1))             (solver.py:323)
+// STAGE 3: U  = (1/3) U0 + (2/3) (U2 + dt L(U2)), psi decay (solver.py:327-333)
+// followed by the scalar bounds of solver.py:199-204 for stages 1-3.
+template <class real, class P, int STAGE, bool POWELL>
+__global__ __launch_bounds__(WAVE* WPB) void stage_kernel(StageParams<real> a) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int wave = xcd_logical_block() * WPB + (threadIdx.x >> 6);
+    if (wave >= a.n_strips * a.n_segs) return;
+    const int strip = wave % a.n_strips;
+    const int seg = wave / a.n_strips;
+
+    double dt_d = a.dt, ch_d = a.ch, decay_d = a.decay;
+    if (a.ctl) {
+        if (a.ctl->done) return;
+        dt_d = a.ctl->dt; ch_d = a.ctl->ch; decay_d = a.ctl->decay;
+    }
+    const FluxConst<real> kc = make_flux_const<real>(a.gamma, ch_d);
+    const real dtr = real(dt_d);
+    const real dxr = real(a.dx), dyr = real(a.dy);
+    const real idx = real(1.0 / a.dx), idy = real(1.0 / a.dy);
+
+    const int nx = a.nx, ny = a.ny;
+    const long plane = a.plane;
+    const int j = strip * STRIP - HALO + lane;
+    const int sj = remap(j, ny, a.bcy_periodic);
+    const bool out_lane = lane >= HALO && lane < WAVE - HALO && j < ny;
+    const int i0 = seg * a.rows;
+    const int i1 = min(i0 + a.rows, nx);
+    const bool psi_x_ghost_zero = !a.bcx_periodic;
+
+    auto row_ptr = [&](const real* base, int i) -> const real* {
+        return base + (long)remap(i, nx, a.bcx_periodic) * a.pitch + sj;
+    };
+    auto ghost_row = [&](int i) -> bool { return psi_x_ghost_zero && (i < 0 || i >= nx); };
+
+    real wm1[NV], w0[NV], wp1[NV];
+    real xl[NV], fxl[NV];
+    {
+        real u[NV], wm2[NV], dm1[NV], d0[NV], fa[NV], fb[NV];
+        load_cell(row_ptr(a.us, i0 - 2), plane, u);
+        cons_to_prim<real, P>(u, wm2, kc.gamma, ghost_row(i0 - 2));
+        load_cell(row_ptr(a.us, i0 - 1), plane, u);
+        cons_to_prim<real, P>(u, wm1, kc.gamma, ghost_row(i0 - 1));
+        load_cell(row_ptr(a.us, i0), plane, u);
+        cons_to_prim<real, P>(u, w0, kc.gamma, ghost_row(i0));
+        load_cell(row_ptr(a.us, i0 + 1), plane, u);
+        cons_to_prim<real, P>(u, wp1, kc.gamma, ghost_row(i0 + 1));
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            dm1[v] = vl_slope<real, P>(wm2[v], wm1[v], w0[v]);
+            d0[v] = vl_slope<real, P>(wm1[v], w0[v], wp1[v]);
+            fa[v] = wm1[v] + real(0.5) * dm1[v];
+            fb[v] = w0[v] - real(0.5) * d0[v];
+            xl[v] = w0[v] + real(0.5) * d0[v];
+        }
+        floor_face(fa); floor_face(fb); floor_face(xl);
+        hlld_face<real, P>(fa, fb, fxl, kc);
+    }
+
+    double mx[NPART_DT] = {0.0, 0.0, 0.0, 0.0};
+    double bf_sum[2] = {0.0, 0.0};
+    real upre[NV];
+    load_cell(row_ptr(a.us, i0 + 2), plane, upre);
+
+    for (int i = i0; i < i1; ++i) {
+        real wp2[NV];
+        cons_to_prim<real, P>(upre, wp2, kc.gamma, ghost_row(i + 2));
+        if (i + 1 < i1) load_cell(row_ptr(a.us, i + 3), plane, upre);
+
+        // stage input and base at this row (interior; halo lanes read a
+        // valid remapped address and never store)
+        real ucur[NV], ubase[NV];
+        const long off = (long)i * a.pitch + sj;
+        load_cell(a.us + off, plane, ucur);
+        if constexpr (STAGE >= 2) load_cell(a.u0 + off, plane, ubase);
+
+        // ---- x face i+1/2
+        real fxr[NV];
+        {
+            real xr[NV], d1[NV];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                d1[v] = vl_slope<real, P>(w0[v], wp1[v], wp2[v]);
+                xr[v] = wp1[v] - real(0.5) * d1[v];
+            }
+            floor_face(xr);
+            hlld_face<real, P>(xl, xr, fxr, kc);
+#pragma unroll
+            for (int v = 0; v < NV; ++v) xl[v] = wp1[v] + real(0.5) * d1[v];
+            floor_face(xl);
+        }
+
+        // ---- y face j+1/2 of row i (neighbours through DPP lane shifts)
+        real fy[NV];
+        real by_prev, by_next;
+        {
+            real yl[NV], yr[NV];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                const real wpv = from_prev(w0[v]);
+                const real wnv = from_next(w0[v]);
+                if (v == iBY) { by_prev = wpv; by_next = wnv; }
+                const real d = vl_slope<real, P>(wpv, w0[v], wnv);
+                yl[v] = w0[v] + real(0.5) * d;
+                yr[v] = from_next(w0[v] - real(0.5) * d);
+            }
+            floor_face(yl); floor_face(yr);
+            hlld_face_y<real, P>(yl, yr, fy, kc);
+        }
+
+        // ---- RHS: ((0 - dFx/dx) - dFy/dy) + S   (solver.py:297-306)
+        real divb = real(0);
+        if constexpr (POWELL) {
+            // solver.py:249-254, on the BC-filled primitive field
+            const real half = real(0.5);
+            divb = P::divi(half * (w0[iBX] + wp1[iBX]) - half * (wm1[iBX] + w0[iBX]), dxr, idx) +
+                   P::divi(half * (w0[iBY] + by_next) - half * (by_prev + w0[iBY]), dyr, idy);
+        }
+        real unew[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            const real fyl = from_prev(fy[v]);
+            real r = real(0) - P::divi(fxr[v] - fxl[v], dxr, idx);
+            r = r - P::divi(fy[v] - fyl, dyr, idy);
+            if constexpr (POWELL) {
+                // S = -divB (0, B, v, v.B, 0, 0)   (solver.py:256-264)
+                real s = real(0);
+                if (v == MX) s = -divb * w0[iBX];
+                if (v == MY) s = -divb * w0[iBY];
+                if (v == MZ) s = -divb * w0[iBZ];
+                if (v == BX) s = -divb * w0[iVX];
+                if (v == BY) s = -divb * w0[iVY];
+                if (v == BZ) s = -divb * w0[iVZ];
+                if (v == EN) s = -divb * (w0[iVX] * w0[iBX] + w0[iVY] * w0[iBY] + w0[iVZ] * w0[iBZ]);
+                r = r + s;
+            }
+            if constexpr (STAGE == 0) unew[v] = r;
+            else if constexpr (STAGE == 1) unew[v] = ucur[v] + dtr * r;
+            else if constexpr (STAGE == 2) unew[v] = real(0.75) * ubase[v] + real(0.25) * (ucur[v] + dtr * r);
+            else unew[v] = real(1.0 / 3.0) * ubase[v] + real(2.0 / 3.0) * (ucur[v] + dtr * r);
+        }
+        if constexpr (STAGE >= 1) {
+            // enforce_scalar_bounds (solver.py:199-204)
+            const real rho = unew[RHO] > real(FLOOR_RHO) ? unew[RHO] : real(FLOOR_RHO);
+            real c = P::div(unew[RHOC], rho);
+            c = c < real(0) ? real(0) : (c > real(1) ? real(1) : c);
+            unew[RHOC] = rho * c;
+        }
+        if constexpr (STAGE == 3) {
+            if (ch_d > 0.0) unew[PSI] = unew[PSI] * real(decay_d);
+        }
+
+        if (out_lane) {
+            if constexpr (STAGE == 0) {
+                real* o = a.uo + (long)i * a.opitch + j;
+#pragma unroll
+                for (int v = 0; v < NV; ++v) o[v * a.oplane] = unew[v];
+            } else {
+                real* o = a.uo + off;
+#pragma unroll
+                for (int v = 0; v < NV; ++v) o[v * plane] = unew[v];
+            }
+            if constexpr (STAGE == 3) dt_terms(unew, a.gamma, mx);
+            if constexpr (STAGE == 1) {
+                if (!a.bcx_periodic) {
+                    if (i == 0) bf_sum[0] += bflux_term(ucur, a.gamma, a.dy);
+                    if (i == nx - 1) bf_sum[1] += bflux_term(ucur, a.gamma, a.dy);
+                }
+            }
+        }
+
+        // slide the window
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            wm1[v] = w0[v]; w0[v] = wp1[v]; wp1[v] = wp2[v]; fxl[v] = fxr[v];
+        }
+    }
+
+    if constexpr (STAGE == 3) {
+        if (a.part_dt) {
+            double r[NPART_DT];
+#pragma unroll
+            for (int q = 0; q < NPART_DT; ++q) r[q] = wave_max(mx[q]);
+            if (lane == 0) {
+#pragma unroll
+                for (int q = 0; q < NPART_DT; ++q) a.part_dt[(long)wave * NPART_DT + q] = r[q];
+            }
+        }
+    }
+    if constexpr (STAGE == 1) {
+        if (a.part_bf && !a.bcx_periodic) {
+            const double s0 = wave_sum(bf_sum[0]);
+            const double s1 = wave_sum(bf_sum[1]);
+            if (lane == 0) {
+                if (i0 == 0) a.part_bf[strip * 2 + 0] = s0;
+                if (i1 == nx) a.part_bf[strip * 2 + 1] = s1;
+            }
+        }
+    }
+}
+
+// ------------------------------------------------- CFL partials from state
+// One partial per workgroup over the interior layout (used when no stage-3
+// partials are current, e.g. right after a state upload).
+template <class real>
+__global__ __launch_bounds__(256) void dt_partials_kernel(const real* __restrict__ u, long plane, int pitch,
+                                                          int nx, int ny, double gamma, double* part) {
+    double m[NPART_DT] = {0.0, 0.0, 0.0, 0.0};
+    const long n = (long)nx * ny;
+    for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x) {
+        const long i = k / ny, jj = k - i * ny;
+        real c[NV];
+        load_cell(u + i * pitch + jj, plane, c);
+        dt_terms(c, gamma, m);
+    }
+    __shared__ double sh[NPART_DT][4];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+    for (int q = 0; q < NPART_DT; ++q) {
+        const double r = wave_max(m[q]);
+        if (lane == 0) sh[q][w] = r;
+    }
+    __syncthreads();
+    if (threadIdx.x < NPART_DT) {
+        double r = sh[threadIdx.x][0];
+        for (int q = 1; q < (int)(blockDim.x >> 6); ++q) r = fmax(r, sh[threadIdx.x][q]);
+        part[(long)blockIdx.x * NPART_DT + threadIdx.x] = r;
+    }
+}
+
+// ------------------------------------------------------------- step control
+enum : int { CTL_DT_ONLY = 0, CTL_HOST_STEP_END = 1, CTL_DEVICE_LOOP = 2 };
+
+#ifdef AMHD_API_TU
+// CFL partials from a primitive array W (NV planes of n cells), as
+// compute_dt(W) of solver.py:206-223 reads it (floors re-applied).
+__global__ __launch_bounds__(256) void dt_partials_w_kernel(const double* __restrict__ W, long n, double gamma,
+                                                            double* part) {
+    double m[NPART_DT] = {0.0, 0.0, 0.0, 0.0};
+    for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x) {
+        const double rho = fmax(W[iRHO * n + k], FLOOR_RHO);
+        const double p = fmax(W[iPR * n + k], FLOOR_PR);
+        const double bx = W[iBX * n + k], by = W[iBY * n + k], bz = W[iBZ * n + k];
+        const double b2 = bx * bx + by * by + bz * bz;
+        const double cf = sqrt(gamma * p / rho + b2 / rho);
+        const double ax = fabs(W[iVX * n + k]), ay = fabs(W[iVY * n + k]);
+        m[0] = fmax(m[0], cf);
+        m[1] = fmax(m[1], ax + ay);
+        m[2] = fmax(m[2], ax);
+        m[3] = fmax(m[3], ay);
+    }
+    __shared__ double sh[NPART_DT][4];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+    for (int q = 0; q < NPART_DT; ++q) {
+        const double r = wave_max(m[q]);
+        if (lane == 0) sh[q][w] = r;
+    }
+    __syncthreads();
+    if (threadIdx.x < NPART_DT) {
+        double r = sh[threadIdx.x][0];
+        for (int q = 1; q < (int)(blockDim.x >> 6); ++q) r = fmax(r, sh[threadIdx.x][q]);
+        part[(long)blockIdx.x * NPART_DT + threadIdx.x] = r;
+    }
+}
+
+
+// Reduces CFL partials to (ch, dt) exactly as solver.py:206-223 (all
+// reductions are maxima; since ch >= 1.5 max cf, max(cf, ch) == ch in every
+// cell and max_k (|v_k| + ch)/dx == (max_k |v_k| + ch)/dx, so the four maxima
+// determine dt bit for bit).  Also closes a step: boundary-flux sum and its
+// accumulation (solver.py:316-317), t and step advance (solver.py:481-482).
+__global__ __launch_bounds__(256) void step_ctl_kernel(StepCtl* ctl, const double* part_dt, int n_part,
+                                                       const double* part_bf, int n_bf, double dx, double dy,
+                                                       double host_dt, double cfl, int mode) {
+    __shared__ double sh[NPART_DT][4];
+    __shared__ double shb[2][4];
+    double m[NPART_DT] = {0.0, 0.0, 0.0, 0.0};
+    for (int k = threadIdx.x; k < n_part; k += blockDim.x) {
+#pragma unroll
+        for (int q = 0; q < NPART_DT; ++q) m[q] = fmax(m[q], part_dt[(long)k * NPART_DT + q]);
+    }
+    // boundary flux: strips summed in a fixed order by one wavefront
+    double b0 = 0.0, b1 = 0.0;
+    if (threadIdx.x < 64 && part_bf) {
+        for (int k = threadIdx.x; k < n_bf; k += 64) { b0 += part_bf[2 * k]; b1 += part_bf[2 * k + 1]; }
+    }
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+    for (int q = 0; q < NPART_DT; ++q) {
+        const double r = wave_max(m[q]);
+        if (lane == 0) sh[q][w] = r;
+    }
+    b0 = wave_sum(b0);
+    b1 = wave_sum(b1);
+    if (lane == 0) { shb[0][w] = b0; shb[1][w] = b1; }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    double r[NPART_DT];
+    for (int q = 0; q < NPART_DT; ++q) {
+        r[q] = sh[q][0];
+        for (int x = 1; x < (int)(blockDim.x >> 6); ++x) r[q] = fmax(r[q], sh[q][x]);
+    }
+    const double bflux = part_bf ? (shb[0][0] - shb[1][0]) : 0.0;
+
+    if (mode == CTL_HOST_STEP_END) {
+        ctl->bflux = bflux;
+        ctl->bflux_cum += bflux * host_dt;
+        return;
+    }
+    if (mode == CTL_DEVICE_LOOP && ctl->pending) {
+        ctl->bflux = bflux;
+        ctl->bflux_cum += bflux * ctl->dt;
+        ctl->t += ctl->dt;
+        ctl->steps += 1;
+        ctl->pending = 0;
+    }
+    if (mode == CTL_DEVICE_LOOP && ctl->done) return;
+
+    const double ch = fmax(r[0] + r[1], 1.0) * 1.5;
+    const double sx = r[2] + ch, sy = r[3] + ch;
+    const double sm = fmax(sx / dx, sy / dy);
+    double dt = (sm < 1e-14) ? cfl * fmin(dx, dy) : cfl / sm;
+    ctl->cfmax = r[0];
+    ctl->vmax = r[1];
+    ctl->ch = ch;
+    if (mode == CTL_DT_ONLY) {
+        ctl->dt = dt;
+        return;
+    }
+    // device loop: solver.py:475-479
+    if (!(ctl->t < ctl->t_end && ctl->steps < ctl->max_steps)) { ctl->done = 1; return; }
+    const double rem = ctl->t_end - ctl->t;
+    if (rem < dt) dt = rem;
+    if (dt <= 1e-16) { ctl->done = 1; return; }
+    ctl->dt = dt;
+    ctl->decay = ch > 0.0 ? exp(-ctl->glm_alpha * ch * dt / ctl->min_dxdy) : 1.0;
+    ctl->pending = 1;
+}
+
+#endif  // AMHD_API_TU
+
+// -------------------------------------------------------- layout transfers
+// Ghosted f64 (NV, nx+4, ny+4) -> interior layout (any real).
+template <class real>
+__global__ void ingest_kernel(const double* __restrict__ ug, real* __restrict__ u, long plane, int pitch, int nx,
+                              int ny) {
+    const int nyt = ny + 4;
+    const long gplane = (long)(nx + 4) * nyt;
+    const long n = (long)nx * ny;
+    for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x) {
+        const long i = k / ny, jj = k - i * ny;
+        const long src = (i + 2) * nyt + (jj + 2);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) u[v * plane + i * pitch + jj] = real(ug[v * gplane + src]);
+    }
+}
+
+// Interior layout -> ghosted f64 with ghosts filled as solver.py:158-197
+// leaves them (images of interior cells; psi = 0 on non-periodic x ghosts).
+template <class real>
+__global__ void materialize_kernel(const real* __restrict__ u, long plane, int pitch, int nx, int ny,
+                                   int bcx_periodic, int bcy_periodic, double* __restrict__ ug) {
+    const int nxt = nx + 4, nyt = ny + 4;
+    const long gplane = (long)nxt * nyt;
+    for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < gplane; k += (long)gridDim.x * blockDim.x) {
+        const int gi = (int)(k / nyt), gj = (int)(k - (long)gi * nyt);
+        const int i = gi - 2, jj = gj - 2;
+        const int si = remap(i, nx, bcx_periodic), sj = remap(jj, ny, bcy_periodic);
+        const bool xg = (i < 0 || i >= nx);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            double x = double(u[v * plane + (long)si * pitch + sj]);
+            if (v == PSI && xg && !bcx_periodic) x = 0.0;
+            ug[v * gplane + k] = x;
+        }
+    }
+}
+
+#ifdef AMHD_API_TU
+// ------------------------------------------------ standalone (drop-in API)
+// physics.py:284-301 over (NV, n)
+__global__ void cons_to_prim_kernel(const double* __restrict__ U, double* __restrict__ W, long n, double gamma) {
+    for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x) {
+        double u[NV], w[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) u[v] = U[v * n + k];
+        cons_to_prim<double, ExactMath<double>>(u, w, gamma, false);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) W[v * n + k] = w[v];
+    }
+}
+
+// physics.py:304-320 over (NV, n)
+__global__ void prim_to_cons_kernel(const double* __restrict__ W, double* __restrict__ U, long n, double gamma) {
+    for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x) {
+        const double rho = fmax(W[iRHO * n + k], FLOOR_RHO);
+        const double vx = W[iVX * n + k], vy = W[iVY * n + k], vz = W[iVZ * n + k];
+        const double bx = W[iBX * n + k], by = W[iBY * n + k], bz = W[iBZ * n + k];
+        U[RHO * n + k] = rho;
+        U[MX * n + k] = rho * vx;
+        U[MY * n + k] = rho * vy;
+        U[MZ * n + k] = rho * vz;
+        U[BX * n + k] = bx;
+        U[BY * n + k] = by;
+        U[BZ * n + k] = bz;
+        U[PSI * n + k] = W[iPSI * n + k];
+        U[RHOC * n + k] = rho * W[iCLR * n + k];
+        const double ke = 0.5 * rho * (vx * vx + vy * vy + vz * vz);
+        const double me = 0.5 * (bx * bx + by * by + bz * bz);
+        U[EN * n + k] = W[iPR * n + k] / (gamma - 1.0) + ke + me;
+    }
+}
+
+// physics.py:323-350.  dir 0: faces along x, output (NV, nxt-3, nyt);
+// dir 1: along y, output (NV, nxt, nyt-3).
+__global__ void muscl_kernel(const double* __restrict__ W, int nxt, int nyt, int dir, double* __restrict__ WL,
+                             double* __restrict__ WR) {
+    const int fx = dir == 0 ? nxt - 3 : nxt;
+    const int fy = dir == 0 ? nyt : nyt - 3;
+    const long nf = (long)fx * fy, plane = (long)nxt * nyt;
+    const long st = dir == 0 ? nyt : 1;
+    for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < nf; k += (long)gridDim.x * blockDim.x) {
+        const long a = k / fy, b = k - a * fy;
+        const long c0 = dir == 0 ? a * nyt + b : a * nyt + b;  // first of the 4 cells
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            const double* w = W + v * plane + c0;
+            const double d1 = vl_slope<double, ExactMath<double>>(w[0], w[st], w[2 * st]);
+            const double d2 = vl_slope<double, ExactMath<double>>(w[st], w[2 * st], w[3 * st]);
+            double l = w[st] + 0.5 * d1;
+            double r = w[2 * st] - 0.5 * d2;
+            if (v == iRHO || v == iPR) {
+                const double fl = v == iRHO ? FLOOR_RHO : FLOOR_PR;
+                l = fmax(l, fl);
+                r = fmax(r, fl);
+            }
+            WL[v * nf + k] = l;
+            WR[v * nf + k] = r;
+        }
+    }
+}
+
+#endif  // AMHD_API_TU
+
+// physics.py:43-275 + 353-379 over (NV, n) face arrays; smax as the bit
+// pattern of a non-negative double (ordered like the value) for atomicMax.
+template <class real, class P>
+__global__ __launch_bounds__(256) void hlld_kernel(const real* __restrict__ WL, const real* __restrict__ WR,
+                                                   real* __restrict__ F, long n, int dir, FluxConst<real> kc,
+                                                   unsigned long long* smax_bits) {
+    double smax = 0.0;
+    for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x) {
+        real wl[NV], wr[NV], f[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) { wl[v] = WL[v * n + k]; wr[v] = WR[v * n + k]; }
+        const real s = dir == 0 ? hlld_face<real, P>(wl, wr, f, kc) : hlld_face_y<real, P>(wl, wr, f, kc);
+        smax = fmax(smax, double(s));
+#pragma unroll
+        for (int v = 0; v < NV; ++v) F[v * n + k] = f[v];
+    }
+    if (smax_bits) {
+        smax = wave_max(smax);
+        if ((threadIdx.x & 63) == 0) atomicMax(smax_bits, (unsigned long long)__double_as_longlong(smax));
+    }
+}
+
+#ifdef AMHD_API_TU
+// solver.py:158-197 in place on a ghosted f64 array.  Every ghost value is
+// an image of an interior cell, so one pass over the ghost cells suffices.
+__global__ void apply_bc_kernel(double* __restrict__ U, int nx, int ny, int bcx_periodic, int bcy_periodic) {
+    const int nxt = nx + 4, nyt = ny + 4;
+    const long gplane = (long)nxt * nyt;
+    for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < gplane; k += (long)gridDim.x * blockDim.x) {
+        const int gi = (int)(k / nyt), gj = (int)(k - (long)gi * nyt);
+        const int i = gi - 2, jj = gj - 2;
+        const bool xg = (i < 0 || i >= nx), yg = (jj < 0 || jj >= ny);
+        if (!xg && !yg) continue;
+        const int si = remap(i, nx, bcx_periodic) + 2, sj = remap(jj, ny, bcy_periodic) + 2;
+        const long src = (long)si * nyt + sj;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            double x = U[v * gplane + src];
+            if (v == PSI && xg && !bcx_periodic) x = 0.0;
+            U[v * gplane + k] = x;
+        }
+    }
+}
+
+// solver.py:199-204 over (NV, n)
+__global__ void enforce_bounds_kernel(double* __restrict__ U, long n) {
+    for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x) {
+        const double rho = fmax(U[RHO * n + k], FLOOR_RHO);
+        double c = U[RHOC * n + k] / rho;
+        c = c < 0.0 ? 0.0 : (c > 1.0 ? 1.0 : c);
+        U[RHOC * n + k] = rho * c;
+    }
+}
+
+// psi *= decay over a plane (solver.py:333)
+__global__ void scale_plane_kernel(double* __restrict__ x, long n, double s) {
+    for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x) x[k] *= s;
+}
+
+#endif  // AMHD_API_TU
+
+}  // namespace amhd

@@ -1,0 +1,81 @@
+// amhd_launch.hpp -- host-side launchers shared by the precision TUs.
+//
+// amhd_f64.hip instantiates them for double with ExactMath (built with
+// -ffp-contract=off: the reference's IEEE FP64 arithmetic), amhd_f32.hip for
+// float with FastMath (contraction allowed).  amhd_api.hip only sees the
+// declarations below.
+#pragma once
+#include "amhd_kernels.hpp"
+
+namespace amhd {
+
+// One SSP-RK3 stage (0 = RHS only) of the fused strip-march kernel.
+hipError_t launch_stage_f64(int stage, bool powell, const StageParams<double>& a, hipStream_t s);
+hipError_t launch_stage_f32(int stage, bool powell, const StageParams<float>& a, hipStream_t s);
+
+// CFL partial maxima over an interior-layout state; returns #partials.
+int launch_dt_partials_f64(const double* u, long plane, int pitch, int nx, int ny, double gamma, double* part,
+                           int max_parts, hipStream_t s);
+int launch_dt_partials_f32(const float* u, long plane, int pitch, int nx, int ny, double gamma, double* part,
+                           int max_parts, hipStream_t s);
+
+hipError_t launch_ingest_f64(const double* ug, double* u, long plane, int pitch, int nx, int ny, hipStream_t s);
+hipError_t launch_ingest_f32(const double* ug, float* u, long plane, int pitch, int nx, int ny, hipStream_t s);
+hipError_t launch_materialize_f64(const double* u, long plane, int pitch, int nx, int ny, int bcx, int bcy,
+                                  double* ug, hipStream_t s);
+hipError_t launch_materialize_f32(const float* u, long plane, int pitch, int nx, int ny, int bcx, int bcy,
+                                  double* ug, hipStream_t s);
+
+hipError_t launch_hlld_f32(const float* wl, const float* wr, float* f, long n, int dir, double gamma, double ch,
+                           double floor_rho, double floor_pr, unsigned long long* smax_bits, hipStream_t s);
+
+inline int grid_for(long n, int block, int cap = 4096) {
+    long g = (n + block - 1) / block;
+    if (g < 1) g = 1;
+    return (int)(g > cap ? cap : g);
+}
+
+// Template bodies, included by the two precision TUs only.
+#ifdef AMHD_LAUNCH_IMPL
+template <class real, class P>
+hipError_t launch_stage_impl(int stage, bool powell, const StageParams<real>& a, hipStream_t s) {
+    const int n_waves = a.n_strips * a.n_segs;
+    const dim3 grid((n_waves + WPB - 1) / WPB), block(WAVE * WPB);
+    switch (stage * 2 + (powell ? 1 : 0)) {
+        case 0: stage_kernel<real, P, 0, false><<<grid, block, 0, s>>>(a); break;
+        case 1: stage_kernel<real, P, 0, true><<<grid, block, 0, s>>>(a); break;
+        case 2: stage_kernel<real, P, 1, false><<<grid, block, 0, s>>>(a); break;
+        case 3: stage_kernel<real, P, 1, true><<<grid, block, 0, s>>>(a); break;
+        case 4: stage_kernel<real, P, 2, false><<<grid, block, 0, s>>>(a); break;
+        case 5: stage_kernel<real, P, 2, true><<<grid, block, 0, s>>>(a); break;
+        case 6: stage_kernel<real, P, 3, false><<<grid, block, 0, s>>>(a); break;
+        case 7: stage_kernel<real, P, 3, true><<<grid, block, 0, s>>>(a); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <class real>
+int launch_dt_partials_impl(const real* u, long plane, int pitch, int nx, int ny, double gamma, double* part,
+                            int max_parts, hipStream_t s) {
+    const int g = grid_for((long)nx * ny, 256, max_parts);
+    dt_partials_kernel<real><<<g, 256, 0, s>>>(u, plane, pitch, nx, ny, gamma, part);
+    return hipGetLastError() == hipSuccess ? g : -1;
+}
+
+template <class real>
+hipError_t launch_ingest_impl(const double* ug, real* u, long plane, int pitch, int nx, int ny, hipStream_t s) {
+    ingest_kernel<real><<<grid_for((long)nx * ny, 256), 256, 0, s>>>(ug, u, plane, pitch, nx, ny);
+    return hipGetLastError();
+}
+
+template <class real>
+hipError_t launch_materialize_impl(const real* u, long plane, int pitch, int nx, int ny, int bcx, int bcy,
+                                   double* ug, hipStream_t s) {
+    materialize_kernel<real><<<grid_for((long)(nx + 4) * (ny + 4), 256), 256, 0, s>>>(u, plane, pitch, nx, ny, bcx,
+                                                                                        bcy, ug);
+    return hipGetLastError();
+}
+#endif
+
+}  // namespace amhd

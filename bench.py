@@ -1,0 +1,201 @@
+#!/usr/bin/env python3
+"""Throughput of the gfx950 MHD update on one GPU.
+
+Workload (default): the magnetic-sieve RMI on a 2048 x 1024 grid in FP32 --
+MUSCL/van Leer + HLLD + GLM psi transport + Powell source + SSP-RK3 + CFL
+reduction, dt computed on the device every step.  A "step" is one full
+SSP-RK3 step of the whole grid (three fused stage launches plus the step
+control launch).  Inputs are resident in HBM before the timed region.
+
+Prints ONE JSON line: Mcell-updates/s, the roofline of the fused stage
+kernel (algorithmic bytes 8 * 10 * sizeof(real) per cell-update divided by
+the stage kernels' device time, HIP events on the context stream over the
+timed region), and the CPU oracle timed on this host on a bounded sample.
+
+  python bench.py [--steps K] [--warmup W] [--workload sieve2048|rmi1024-fp64|rmi1024-fp32]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "aeromhd-cuda_amd"))
+
+METRIC = "Mcell-updates/s per RK3 step + % HBM roofline; closure speedup at equal L1 err"
+HBM_PEAK_GBS = 8000.0
+NVAR = 10
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_workload(name: str, seed: int):
+    from amhd_ics import magnetic_sieve, rmi_uniform_bx
+
+    if name == "sieve2048":
+        cfg, W = magnetic_sieve(2048, 1024, precision="fp32", seed=seed)
+        desc = f"magnetic-sieve RMI 2048x1024 FP32 (seed {seed}): MUSCL+HLLD+GLM+Powell, SSP-RK3, device dt"
+    elif name in ("rmi1024-fp64", "rmi1024-fp32"):
+        prec = name.split("-")[1]
+        cfg, W = rmi_uniform_bx(1024, 512, precision=prec)
+        desc = f"single-mode RMI 1024x512 {prec.upper()} (Mach 10, A=0.5, beta=10 Bx): SSP-RK3, device dt"
+    else:
+        raise SystemExit(f"unknown workload {name}")
+    cfg.t_end = 1e30
+    cfg.max_steps = 2 ** 30
+    return cfg, W, desc
+
+
+def gpu_measure(name: str, steps: int, warmup: int, seed: int, rows: int | None):
+    import torch
+    from physics import prim_to_cons
+    from solver import MHDSolver
+
+    cfg, W, desc = make_workload(name, seed)
+    s = MHDSolver(cfg)
+    if rows:
+        s.ctx.set_rows(rows)
+    s.U = prim_to_cons(W, cfg.gamma)
+    s.advance(warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    t, step, last, stage_ms, ctl_ms = s.ctx.run_profiled(steps, cfg.cfl, cfg.t_end, cfg.max_steps, s.t, s.step)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    assert step - s.step == steps, "device loop stopped early"
+    s.t, s.step = t, step
+    # graph-replayed loop of the same length (launch overhead removed)
+    torch.cuda.synchronize()
+    g0 = time.perf_counter()
+    n = s.advance(steps, use_graph=True)
+    torch.cuda.synchronize()
+    gwall = time.perf_counter() - g0
+    cells = cfg.nx * cfg.ny
+    esize = 4 if cfg.precision == "fp32" else 8
+    bytes_step = 8 * NVAR * esize * cells
+    kern_s = sum(stage_ms) * 1e-3 / steps
+    sums = s.ctx.global_sums()
+    return {
+        "desc": desc, "cfg": cfg, "cells": cells, "esize": esize, "rows": s.ctx.rows,
+        "mcups": cells * steps / wall / 1e6, "ms_per_step": wall * 1e3 / steps,
+        "graph_mcups": cells * n / gwall / 1e6,
+        "stage_avg_us": [1e3 * m / steps for m in stage_ms], "ctl_avg_us": 1e3 * ctl_ms / steps,
+        "achieved_gbs": bytes_step / kern_s / 1e9, "bytes_per_step": bytes_step,
+        "bytes_per_launch": [2 * NVAR * esize * cells, 3 * NVAR * esize * cells, 3 * NVAR * esize * cells],
+        "finite": all(x == x for x in sums), "t": t, "dt": last,
+    }
+
+
+def cpu_baseline(seed: int, steps: int, threads: int):
+    """The CPU oracle (C restatement of the reference math, FP64, OpenMP
+    over x rows) on the 2048 x 1024 sieve: `steps` SSP-RK3 steps with the
+    reference's per-step dt."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle
+    from amhd_ics import cons_from_prim_host, magnetic_sieve
+
+    oracle.build()
+    oracle.set_threads(threads)
+    cfg, W = magnetic_sieve(2048, 1024, precision="fp64", seed=seed)
+    cfg.t_end = 1e30
+    U = cons_from_prim_host(W, cfg.gamma)
+    U, *_ = oracle.run_loop(U, cfg, 1)          # untimed first step (page-in)
+    t0 = time.perf_counter()
+    U, _, n, _, _ = oracle.run_loop(U, cfg, steps)
+    wall = time.perf_counter() - t0
+    return {"value": round(cfg.nx * cfg.ny * n / wall / 1e6, 3), "unit": "Mcell-updates/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{n} SSP-RK3 steps (with per-step CFL dt) of the 2048x1024 magnetic-sieve RMI in FP64 "
+                      f"(oracle/oracle.c, gcc -O3 -fopenmp, {threads} threads), {wall:.1f} s"}
+
+
+def traffic_from_profile(workload: str):
+    """HBM bytes per stage launch measured by rocprofv3 FETCH_SIZE /
+    WRITE_SIZE (profiles/stage_kernel_traffic_<workload>.json), or None."""
+    path = os.path.join(REPO, "profiles", f"stage_kernel_traffic_{workload}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("bytes_per_launch")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="sieve2048")
+    ap.add_argument("--rows", type=int, default=0)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--cpu-steps", type=int, default=6)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-secondary", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
+              "(the single-instance solver does not shard across GPUs).", file=sys.stderr)
+        sys.exit(2)
+
+    log(f"[bench] workload {args.workload}: warmup {args.warmup}, steps {args.steps}")
+    m = gpu_measure(args.workload, args.steps, args.warmup, args.seed, args.rows or None)
+    log(f"[bench] {m['mcups']:.1f} Mcell/s, stage us {m['stage_avg_us']}, achieved {m['achieved_gbs']:.0f} GB/s")
+
+    secondary = []
+    if not args.no_secondary:
+        for wl in ("rmi1024-fp64", "rmi1024-fp32"):
+            if wl == args.workload:
+                continue
+            log(f"[bench] secondary {wl}")
+            x = gpu_measure(wl, 200, 20, args.seed, None)
+            secondary.append({"workload": x["desc"], "value": round(x["mcups"], 2), "unit": "Mcell-updates/s",
+                              "ms_per_step": round(x["ms_per_step"], 4),
+                              "achieved_GBps": round(x["achieved_gbs"], 1),
+                              "frac": round(x["achieved_gbs"] / HBM_PEAK_GBS, 4),
+                              "stage_avg_us": [round(u, 2) for u in x["stage_avg_us"]]})
+
+    cpu = None
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+        log(f"[bench] cpu baseline: oracle, {threads} threads, {args.cpu_steps} steps")
+        cpu = cpu_baseline(args.seed, args.cpu_steps, threads)
+
+    traffic = traffic_from_profile(args.workload)
+    avg_launch_bytes = sum(m["bytes_per_launch"]) / 3
+    out = {
+        "metric": METRIC,
+        "value": round(m["mcups"], 2),
+        "unit": "Mcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(m["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32" if m["esize"] == 4 else "f64",
+        "data": "synthetic (seeded initial condition generated on the host)",
+        "config": {"workload": m["desc"], "nx": m["cfg"].nx, "ny": m["cfg"].ny, "rows_per_segment": m["rows"],
+                   "cfl": m["cfg"].cfl, "powell": bool(m["cfg"].powell_source and m["cfg"].B_transverse > 0)},
+        "roofline": {"bound": "hbm", "achieved": round(m["achieved_gbs"], 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(m["achieved_gbs"] / HBM_PEAK_GBS, 4),
+                     "traffic": traffic,
+                     "kernel": "stage_kernel (3 launches per step)",
+                     "algorithmic_bytes_per_launch": [int(b) for b in m["bytes_per_launch"]],
+                     "avg_launch_bytes": int(avg_launch_bytes),
+                     "stage_avg_us": [round(u, 2) for u in m["stage_avg_us"]],
+                     "ctl_avg_us": round(m["ctl_avg_us"], 2)},
+        "cpu_baseline": cpu,
+        "graph_replay_value": round(m["graph_mcups"], 2),
+        "secondary": secondary,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,142 @@
+/*
+ * aeromhd.h -- C ABI of libaeromhd_hip.so, the gfx950 implementation of the
+ * 2-D ideal-MHD update of Semedo2002/AeroMHD-CUDA (MUSCL / van Leer
+ * reconstruction, HLLD flux, GLM psi transport with Powell source, SSP-RK3
+ * stage combine, CFL max-wavespeed reduction).
+ *
+ * Conventions
+ *   - Every function returns 0 on success and a negative code on failure;
+ *     amhd_last_error() then describes the failure (thread-local text).
+ *       -1 invalid argument, -2 HIP runtime error, -3 out of memory,
+ *       -4 unsupported configuration.
+ *   - State arrays crossing the ABI have the reference's layout: the ghosted
+ *     conserved array U of shape (10, nx+4, ny+4), C order, float64, with
+ *     variable order [rho, mx, my, mz, Bx, By, Bz, E, psi, rhoC]
+ *     (config.py:13-16); primitive arrays use [rho, vx, vy, vz, Bx, By, Bz,
+ *     p, psi, C] (config.py:19-22).
+ *   - `stream` arguments are hipStream_t handles passed as void* (NULL = the
+ *     null stream).  Functions that return a host value synchronise.
+ *   - A context (amhd_ctx) owns its device buffers and its own stream.  It is
+ *     not thread-safe; use one context per thread.
+ */
+#ifndef AEROMHD_H
+#define AEROMHD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AMHD_ABI_VERSION 1
+
+enum { AMHD_F64 = 0, AMHD_F32 = 1 };
+
+/* Solver parameters: the fields of config.Config (config.py:47-89) that the
+ * update reads.  bc_*_periodic: 1 for "periodic", 0 for "extrapolation" and
+ * "characteristic" (identical in effect, solver.py:167-176).  powell must
+ * already be (powell_source && B_transverse > 0), the gate of
+ * solver.py:304. */
+typedef struct {
+    double gamma;
+    double dx, dy;
+    double x_min, y_min;
+    double glm_alpha;
+    int bc_x_periodic;
+    int bc_y_periodic;
+    int powell;
+    int reserved;
+} amhd_params;
+
+const char* amhd_last_error(void);
+int amhd_abi_version(void);
+
+/* ---- solver context: replaces MHDSolver's device state (solver.py:72-87) */
+
+/* Allocate a context for an nx x ny interior grid (ng = 2, solver.py:74).
+ * dtype: AMHD_F64 (reference arithmetic) or AMHD_F32. */
+int amhd_create(void** ctx, int nx, int ny, int dtype, const amhd_params* p);
+int amhd_destroy(void* ctx);
+
+/* Rows per wavefront segment of the fused stage kernel (0 = default). */
+int amhd_set_rows(void* ctx, int rows);
+int amhd_get_rows(void* ctx);
+
+/* MHDSolver.U assignment / read (solver.py:83, 139).  U: ghosted float64
+ * (10, nx+4, ny+4), host or device memory.  On read, ghost cells are filled
+ * as apply_bc (solver.py:158-197) leaves them. */
+int amhd_upload_state(void* ctx, const double* U, int on_device);
+int amhd_download_state(void* ctx, double* U, int on_device);
+
+/* MHDSolver.compute_dt on the context's own state (solver.py:206-223):
+ * returns dt and glm_ch (the value the reference stores in cfg.glm_ch). */
+int amhd_compute_dt(void* ctx, double cfl, double* dt, double* ch);
+
+/* MHDSolver.step_ssprk3 (solver.py:310-333) with self.dt = dt and the
+ * frozen cfg.glm_ch = ch.  psi_decay = exp(-glm_alpha*ch*dt/min(dx,dy)),
+ * evaluated by the caller as the reference does on the host
+ * (solver.py:331-333).  Asynchronous on the context stream. */
+int amhd_step_ssprk3(void* ctx, double dt, double ch, double psi_decay);
+
+/* Device-resident time loop of MHDSolver.run (solver.py:475-482) without
+ * diagnostics: up to n_steps steps, stopping at t_end or max_steps (counted
+ * in *step_inout) like the reference loop.  dt, ch and the psi decay are
+ * computed on the device; no host round trip per step.  use_graph: replay
+ * a captured hipGraph of the step sequence.  Synchronises at the end. */
+int amhd_run(void* ctx, int n_steps, double cfl, double t_end, int max_steps, int use_graph, double* t_inout,
+             int* step_inout, double* last_dt);
+
+/* amhd_run without graph replay and with a HIP event pair around every
+ * launch on the context stream: stage_ms[s] receives the summed device time
+ * of the stage-(s+1) kernel over the steps run, *ctl_ms that of the step
+ * control kernel (may be NULL).  For measurement. */
+int amhd_run_profiled(void* ctx, int n_steps, double cfl, double t_end, int max_steps, double* t_inout,
+                      int* step_inout, double* last_dt, double stage_ms[3], double* ctl_ms);
+
+/* Accumulated boundary energy flux (solver.py:316-317) and the last step's
+ * flux. */
+int amhd_boundary_energy(void* ctx, double* cumulative, double* last);
+int amhd_reset_boundary_energy(void* ctx, double value);
+
+/* MHDSolver.compute_rhs(U) (solver.py:268-308) for a ghosted float64 device
+ * array U, which receives apply_bc in place as in the reference; R (same
+ * shape, device) is zero in the ghosts.  ch: the frozen glm_ch. */
+int amhd_compute_rhs(void* ctx, double* U_dev, double* R_dev, double ch);
+
+/* FP64 global sums of the interior state: out[0] = sum(rho)*dx*dy,
+ * out[1] = sum(E)*dx*dy (solver.py:430), out[2] = max|divB|,
+ * out[3] = sqrt(mean(divB^2)) with divB as solver.py:419-428. */
+int amhd_global_sums(void* ctx, double out[4]);
+
+/* ---- standalone operators of physics.py / solver.py (float64, device
+ * pointers, asynchronous on `stream` unless a host result is returned) */
+
+/* physics.py:284-301 / 304-320 on (10, n) planes. */
+int amhd_cons_to_prim(const double* U, double* W, long n, double gamma, void* stream);
+int amhd_prim_to_cons(const double* W, double* U, long n, double gamma, void* stream);
+
+/* physics.py:323-350: W (10, nxt, nyt) -> WL, WR of shape (10, nxt-3, nyt)
+ * (dir 0, muscl_x) or (10, nxt, nyt-3) (dir 1, muscl_y). */
+int amhd_muscl(const double* W, int nxt, int nyt, int dir, double* WL, double* WR, void* stream);
+
+/* The HLLD kernel hlld_flux_x_kernel(WL, WR, F, smax_arr, n_faces, gamma,
+ * ch, floor_rho, floor_pr) of physics.py:44-55 with the wrappers of
+ * physics.py:353-379: (10, n_faces) planes, dir 0 = x faces, dir 1 = y faces
+ * (vx<->vy, Bx<->By rotation).  smax (may be NULL) receives
+ * max(|SL|,|SR|) over the faces, as float(xp.max(smax_arr)) does
+ * (synchronises).  dtype AMHD_F64 or AMHD_F32 for WL/WR/F. */
+int amhd_hlld_flux(const void* WL, const void* WR, void* F, double* smax, long n_faces, int dir, double gamma,
+                   double ch, double floor_rho, double floor_pr, int dtype, void* stream);
+
+/* solver.py:158-197 in place on a ghosted (10, nx+4, ny+4) device array. */
+int amhd_apply_bc(double* U, int nx, int ny, int bc_x_periodic, int bc_y_periodic, void* stream);
+
+/* solver.py:199-204 in place on (10, n) planes. */
+int amhd_enforce_scalar_bounds(double* U, long n, void* stream);
+
+/* solver.py:206-223 over a primitive (10, n) device array W: dt and ch. */
+int amhd_compute_dt_w(const double* W, long n, double gamma, double cfl, double dx, double dy, double* dt,
+                      double* ch, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* AEROMHD_H */
