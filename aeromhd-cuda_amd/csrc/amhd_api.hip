@@ -69,7 +69,8 @@ int get_scratch(Scratch** out) {
 struct Ctx {
     int nx = 0, ny = 0, dtype = AMHD_F64;
     amhd_params p{};
-    int rows = 16, n_strips = 0, n_segs = 0;
+    int rows = 0, n_strips = 0, n_segs = 0;   // rows: requested rows per wave (0 = auto)
+    int slots = 2048;                          // resident stage-kernel wavefronts on the device
     long plane = 0;
     int pitch = 0;
     void* buf[3] = {nullptr, nullptr, nullptr};
@@ -83,17 +84,18 @@ struct Ctx {
     StepCtl* ctl = nullptr;
     hipStream_t stream = nullptr;
     hipGraphExec_t graph[3] = {nullptr, nullptr, nullptr};
-    int graph_rows = -1;
 };
 
 size_t esize(int dtype) { return dtype == AMHD_F32 ? sizeof(float) : sizeof(double); }
 
+// Strips of STRIP y cells; x rows split into n_segs balanced segments.
+// rows == 0: one segment per strip and resident wavefront slot, so that
+// every wave of a stage launch runs concurrently with equal work.
 void set_tiling(Ctx* c, int rows) {
-    if (rows <= 0) rows = 16;
-    if (rows > c->nx) rows = c->nx;
     c->rows = rows;
     c->n_strips = (c->ny + STRIP - 1) / STRIP;
-    c->n_segs = (c->nx + rows - 1) / rows;
+    int segs = rows > 0 ? (c->nx + rows - 1) / rows : c->slots / c->n_strips;
+    c->n_segs = std::max(1, std::min(segs, c->nx));
 }
 
 int n_waves(const Ctx* c) { return c->n_strips * c->n_segs; }
@@ -106,7 +108,8 @@ StageParams<real> stage_args(const Ctx* c, int stage, const real* u0, const real
     a.pitch = c->pitch; a.opitch = c->pitch;
     a.nx = c->nx; a.ny = c->ny;
     a.bcx_periodic = c->p.bc_x_periodic; a.bcy_periodic = c->p.bc_y_periodic;
-    a.n_strips = c->n_strips; a.n_segs = c->n_segs; a.rows = c->rows;
+    a.n_strips = c->n_strips; a.n_segs = c->n_segs;
+    a.in_bytes = a.out_bytes = (unsigned)(esize(c->dtype) * NV * (size_t)c->plane);
     a.gamma = c->p.gamma; a.dx = c->p.dx; a.dy = c->p.dy;
     a.ctl = nullptr;
     a.part_dt = stage == 3 ? c->part_dt : nullptr;
@@ -119,11 +122,11 @@ hipError_t launch_stage_any(const Ctx* c, int stage, const void* u0, const void*
     if (c->dtype == AMHD_F64) {
         auto a = stage_args<double>(c, stage, (const double*)u0, (const double*)us, (double*)uo);
         a.dt = dt; a.ch = ch; a.decay = decay; a.ctl = ctl;
-        return launch_stage_f64(stage, c->p.powell != 0, a, c->stream);
+        return launch_stage_f64(stage, c->p.powell != 0, ctl != nullptr || ch > 0.0, a, c->stream);
     }
     auto a = stage_args<float>(c, stage, (const float*)u0, (const float*)us, (float*)uo);
     a.dt = dt; a.ch = ch; a.decay = decay; a.ctl = ctl;
-    return launch_stage_f32(stage, c->p.powell != 0, a, c->stream);
+    return launch_stage_f32(stage, c->p.powell != 0, ctl != nullptr || ch > 0.0, a, c->stream);
 }
 
 // One SSP-RK3 step on buffers rotating cur -> cur+1 (mod 3).
@@ -225,14 +228,28 @@ int amhd_create(void** out, int nx, int ny, int dtype, const amhd_params* p) {
     c->nx = nx; c->ny = ny; c->dtype = dtype; c->p = *p;
     c->pitch = ny;
     c->plane = (long)nx * ny;
-    set_tiling(c, 16);
+    if ((double)esize(dtype) * NV * (double)c->plane > 4294967295.0 ||
+        (double)sizeof(double) * NV * (double)(nx + 4) * (ny + 4) > 4294967295.0) {
+        delete c;
+        return fail(-4, "amhd_create: %dx%d state exceeds the 4 GiB reach of 32-bit buffer offsets", nx, ny);
+    }
+    {
+        int dev = 0, cus = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && cus > 0) {
+            const int wpc = dtype == AMHD_F64 ? stage_waves_per_cu_f64(p->powell != 0)
+                                             : stage_waves_per_cu_f32(p->powell != 0);
+            c->slots = cus * wpc;
+        }
+    }
+    set_tiling(c, 0);
     const size_t sb = esize(dtype) * NV * (size_t)c->plane;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     for (int k = 0; k < 3 && e == hipSuccess; ++k) e = hipMalloc(&c->buf[k], sb);
     if (e == hipSuccess) e = hipMalloc(&c->staging, sizeof(double) * NV * (size_t)(nx + 4) * (ny + 4));
     // CFL partials: one per wavefront at any tiling (rows >= 1), or one per
     // workgroup of the standalone partials kernel.
-    c->part_cap = std::max((int)((long)nx * ((ny + STRIP - 1) / STRIP)), 4096);
+    c->part_cap = std::max((int)((long)nx * ((ny + STRIP - 1) / STRIP)), 4096);   // >= n_strips * n_segs
     if (e == hipSuccess) e = hipMalloc(&c->part_dt, sizeof(double) * NPART_DT * (size_t)c->part_cap);
     if (e == hipSuccess) e = hipMalloc(&c->part_bf, sizeof(double) * 2 * (size_t)c->n_strips);
     if (e == hipSuccess) e = hipMalloc(&c->gsum, sizeof(double) * 4 * GSUM_BLOCKS);
@@ -274,12 +291,16 @@ int amhd_set_rows(void* ctx, int rows) {
     Ctx* c = (Ctx*)ctx;
     HIP_TRY(hipStreamSynchronize(c->stream));
     set_tiling(c, rows);
-    if (c->graph_rows != c->rows) drop_graphs(c);
+    drop_graphs(c);
     c->part_valid = false;   // per-wave partials follow the tiling
     return 0;
 }
 
-int amhd_get_rows(void* ctx) { return ctx ? ((Ctx*)ctx)->rows : -1; }
+int amhd_get_rows(void* ctx) {
+    if (!ctx) return -1;
+    Ctx* c = (Ctx*)ctx;
+    return (c->nx + c->n_segs - 1) / c->n_segs;
+}
 
 int amhd_upload_state(void* ctx, const double* U, int on_device) {
     if (!ctx || !U) return fail(-1, "amhd_upload_state: null argument");
@@ -363,7 +384,7 @@ int amhd_run(void* ctx, int n_steps, double cfl, double t_end, int max_steps, in
     HIP_TRY(enqueue_ctl(c, c->n_part, 0.0, cfl, CTL_DEVICE_LOOP, c->ctl));
     const int nw = n_waves(c);
     if (use_graph && n_steps > 0) {
-        if (c->graph_rows != c->rows) { drop_graphs(c); c->graph_rows = c->rows; }
+
         hipGraphExec_t& ge = c->graph[c->cur];
         if (!ge) {
             hipGraph_t g;
@@ -504,10 +525,11 @@ int amhd_compute_rhs(void* ctx, double* U, double* R, double ch) {
     a.pitch = a.opitch = nyt;
     a.nx = c->nx; a.ny = c->ny;
     a.bcx_periodic = c->p.bc_x_periodic; a.bcy_periodic = c->p.bc_y_periodic;
-    a.n_strips = c->n_strips; a.n_segs = c->n_segs; a.rows = c->rows;
+    a.n_strips = c->n_strips; a.n_segs = c->n_segs;
+    a.in_bytes = a.out_bytes = (unsigned)(sizeof(double) * ((size_t)NV * gplane - (size_t)(2 * nyt + 2)));
     a.gamma = c->p.gamma; a.dx = c->p.dx; a.dy = c->p.dy;
     a.dt = 0.0; a.ch = ch; a.decay = 1.0;
-    HIP_TRY(launch_stage_f64(0, c->p.powell != 0, a, c->stream));
+    HIP_TRY(launch_stage_f64(0, c->p.powell != 0, ch > 0.0, a, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     return 0;
 }

@@ -34,6 +34,10 @@ template <class real> struct ExactMath {
     static __device__ __forceinline__ real divi(real a, real b, real /*ib*/) { return a / b; }
     static __device__ __forceinline__ real div(real a, real b) { return a / b; }
     static __device__ __forceinline__ real sqrt_(real a) { return sqrt(a); }
+    // Reciprocals of a pair; only the fast policy uses them.
+    static __device__ __forceinline__ void rcp2(real a, real b, real& ia, real& ib) { ia = real(1) / a; ib = real(1) / b; }
+    // sqrt(x) and 1/sqrt(x) for x >= floor (> 0); exact: only sqrt is used.
+    static __device__ __forceinline__ void sqrt_rsq(real x, real& sq, real& rsq) { sq = sqrt(x); rsq = real(0); }
     // epsilons of physics.py:114,117-118,122-123,130
     static __device__ __forceinline__ real eps_den() { return real(1e-14); }
     static __device__ __forceinline__ real eps_sm(real /*sl*/, real /*sr*/) { return real(1e-14); }
@@ -47,6 +51,16 @@ template <> struct FastMath<float> {
     static __device__ __forceinline__ float divi(float a, float /*b*/, float ib) { return a * ib; }
     static __device__ __forceinline__ float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
     static __device__ __forceinline__ float sqrt_(float a) { return __builtin_amdgcn_sqrtf(a); }
+    // one v_rcp_f32 for two reciprocals: r = 1/(a b), 1/a = b r, 1/b = a r
+    static __device__ __forceinline__ void rcp2(float a, float b, float& ia, float& ib) {
+        const float r = __builtin_amdgcn_rcpf(a * b);
+        ia = b * r;
+        ib = a * r;
+    }
+    static __device__ __forceinline__ void sqrt_rsq(float x, float& sq, float& rsq) {
+        rsq = __builtin_amdgcn_rsqf(x);
+        sq = x * rsq;
+    }
     // FP32 cannot resolve the reference's absolute 1e-14 next to O(1..10)
     // speeds: SL + 1e-14 == SL.  The SM clamp therefore uses a margin of
     // 2^-20 of the fan width, and the degenerate-denominator guard a
@@ -124,13 +138,18 @@ __device__ __forceinline__ void cons_to_prim(const real (&u)[NV], real (&w)[NV],
     w[iPR] = p > real(FLOOR_PR) ? p : real(FLOOR_PR);
 }
 
-// van Leer limited slope, physics.py:325-328.
+// van Leer limited slope, physics.py:325-328, from the one-sided
+// differences dm = w0 - wm, dp = wp - w0.
+template <class real, class P>
+__device__ __forceinline__ real vl_slope_d(real dm, real dp) {
+    const real ab = dm * dp;
+    const real d = P::div(real(2) * ab, dm + dp + real(1e-30));
+    return ab > real(0) ? d : real(0);
+}
+
 template <class real, class P>
 __device__ __forceinline__ real vl_slope(real wm, real w0, real wp) {
-    real dm = w0 - wm, dp = wp - w0;
-    real ab = dm * dp;
-    real d = P::div(real(2) * ab, dm + dp + real(1e-30));
-    return ab > real(0) ? d : real(0);
+    return vl_slope_d<real, P>(w0 - wm, wp - w0);
 }
 
 // ---------------------------------------------------------------- HLLD face
@@ -167,7 +186,12 @@ __host__ __device__ inline FluxConst<real> make_flux_const(double gamma, double 
 // is selected first and only F_K, F*_K and F**_K are formed.  Each value is
 // computed with the reference's operation order, so the FP64 result is
 // identical.  Returns max(|SL|, |SR|).
-template <class real, class P>
+//
+// GLM = true: the caller guarantees ch > 0 (always true in the time loop,
+// where ch = 1.5 max(cf + |v|, 1) >= 1.5).  Then SL <= -ch < 0 < ch <= SR,
+// every face lies in the star fan, and f[4], f[8] are the GLM values
+// (physics.py:265-268); the supersonic branches are compiled out.
+template <class real, class P, bool GLM = false>
 __device__ __forceinline__ real hlld_face(const real (&wl)[NV], const real (&wr)[NV], real (&f)[NV],
                                           const FluxConst<real>& k) {
     const real half = real(0.5);
@@ -181,7 +205,7 @@ __device__ __forceinline__ real hlld_face(const real (&wl)[NV], const real (&wr)
     real pR = wr[7] < k.floor_pr ? k.floor_pr : wr[7];
     real psiL = wl[8], psiR = wr[8];
     real cL = wl[9], cR = wr[9];
-    const bool glm = k.ch > real(0);
+    const bool glm = GLM || k.ch > real(0);
 
     // GLM face Bx / psi (physics.py:74-82)
     real bxf, psif;
@@ -199,7 +223,8 @@ __device__ __forceinline__ real hlld_face(const real (&wl)[NV], const real (&wr)
     const real ptR = pR + half * bmR2;
 
     // fast magnetosonic speeds (physics.py:90-102)
-    const real irL = P::rcp(rhoL), irR = P::rcp(rhoR);
+    real irL, irR;
+    P::rcp2(rhoL, rhoR, irL, irR);
     real aL2 = P::divi(k.gamma * pL, rhoL, irL);
     real aR2 = P::divi(k.gamma * pR, rhoR, irR);
     real tmpL = aL2 + P::divi(bmL2, rhoL, irL);
@@ -231,18 +256,21 @@ __device__ __forceinline__ real hlld_face(const real (&wl)[NV], const real (&wr)
     // star states (physics.py:122-154)
     real dslsm = sl - sm; if (fabs(dslsm) < P::eps_den()) dslsm = P::eps_den();
     real dsrsm = sr - sm; if (fabs(dsrsm) < P::eps_den()) dsrsm = P::eps_den();
-    const real idslsm = P::rcp(dslsm), idsrsm = P::rcp(dsrsm);
+    real idslsm, idsrsm;
+    P::rcp2(dslsm, dsrsm, idslsm, idsrsm);
     const real rhoLS = fmax(P::divi(rhoL * dsL, dslsm, idslsm), k.floor_rho);
     const real rhoRS = fmax(P::divi(rhoR * dsR, dsrsm, idsrsm), k.floor_rho);
-    const real sqLS = P::sqrt_(rhoLS);
-    const real sqRS = P::sqrt_(rhoRS);
+    real sqLS, sqRS, rsqLS, rsqRS;
+    P::sqrt_rsq(rhoLS, sqLS, rsqLS);
+    P::sqrt_rsq(rhoRS, sqRS, rsqRS);
 
     const real eps_rel = P::eps_rel_scale() * fmax(fabs(pts), real(1));
     real dnL = rhoL * dsL * dslsm - bx2;
     real dnR = rhoR * dsR * dsrsm - bx2;
     if (fabs(dnL) < eps_rel) dnL = (dnL >= real(0)) ? eps_rel : -eps_rel;
     if (fabs(dnR) < eps_rel) dnR = (dnR >= real(0)) ? eps_rel : -eps_rel;
-    const real idnL = P::rcp(dnL), idnR = P::rcp(dnR);
+    real idnL, idnR;
+    P::rcp2(dnL, dnR, idnL, idnR);
 
     const real vyLS = vyL - P::divi(bxf * byL * (sm - vxL), dnL, idnL);
     const real vzLS = vzL - P::divi(bxf * bzL * (sm - vxL), dnL, idnL);
@@ -265,8 +293,10 @@ __device__ __forceinline__ real hlld_face(const real (&wl)[NV], const real (&wr)
     // double-star state (physics.py:156-172)
     const real sbx = (bxf >= real(0)) ? real(1) : real(-1);
     const real abx = fabs(bxf);
-    real sal = sm - P::div(abx, fmax(sqLS, real(1e-14)));
-    real sar = sm + P::div(abx, fmax(sqRS, real(1e-14)));
+    // |Bx| / max(sqrt(rho*), 1e-14); rho* >= floor_rho so the fast path
+    // multiplies by 1/sqrt(rho*)
+    real sal = sm - P::divi(abx, fmax(sqLS, real(1e-14)), rsqLS);
+    real sar = sm + P::divi(abx, fmax(sqRS, real(1e-14)), rsqRS);
     if (sal < sl) sal = sl;
     if (sal > sm) sal = sm;
     if (sar < sm) sar = sm;
@@ -280,9 +310,9 @@ __device__ __forceinline__ real hlld_face(const real (&wl)[NV], const real (&wr)
     const real vbDS = sm * bxf + vyDS * byDS + vzDS * bzDS;
 
     // region: supersonic L / R, or the star fan split by the contact
-    const bool supL = sl >= real(0);
-    const bool supR = !supL && sr <= real(0);
-    const bool star = !supL && !supR;
+    const bool supL = !GLM && sl >= real(0);
+    const bool supR = !GLM && !supL && sr <= real(0);
+    const bool star = GLM || (!supL && !supR);
     const bool useL = supL || (star && sm >= real(0));
     const bool dbl = star && (useL ? (sal < real(0)) : (sar >= real(0)));
 
@@ -328,6 +358,7 @@ __device__ __forceinline__ real hlld_face(const real (&wl)[NV], const real (&wr)
 
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
+        if (GLM && (v == 4 || v == 8)) continue;   // set below
         real fs = fk[v] + sk * (us[v] - uk[v]);
         real fds;
         if constexpr (P::exact) {
@@ -346,13 +377,13 @@ __device__ __forceinline__ real hlld_face(const real (&wl)[NV], const real (&wr)
 }
 
 // y-face flux: rotate (vx,vy),(Bx,By), solve, rotate back (physics.py:373-379)
-template <class real, class P>
+template <class real, class P, bool GLM = false>
 __device__ __forceinline__ real hlld_face_y(const real (&wl)[NV], const real (&wr)[NV], real (&f)[NV],
                                             const FluxConst<real>& k) {
     real a[NV] = {wl[0], wl[2], wl[1], wl[3], wl[5], wl[4], wl[6], wl[7], wl[8], wl[9]};
     real b[NV] = {wr[0], wr[2], wr[1], wr[3], wr[5], wr[4], wr[6], wr[7], wr[8], wr[9]};
     real g[NV];
-    real s = hlld_face<real, P>(a, b, g, k);
+    real s = hlld_face<real, P, GLM>(a, b, g, k);
     f[0] = g[0]; f[1] = g[2]; f[2] = g[1]; f[3] = g[3]; f[4] = g[5];
     f[5] = g[4]; f[6] = g[6]; f[7] = g[7]; f[8] = g[8]; f[9] = g[9];
     return s;

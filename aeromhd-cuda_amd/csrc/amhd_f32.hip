@@ -6,9 +6,11 @@
 
 namespace amhd {
 
-hipError_t launch_stage_f32(int stage, bool powell, const StageParams<float>& a, hipStream_t s) {
-    return launch_stage_impl<float, FastMath<float>>(stage, powell, a, s);
+hipError_t launch_stage_f32(int stage, bool powell, bool glm, const StageParams<float>& a, hipStream_t s) {
+    return launch_stage_impl<float, FastMath<float>>(stage, powell, glm, a, s);
 }
+
+int stage_waves_per_cu_f32(bool powell) { return stage_waves_per_cu_impl<float, FastMath<float>>(powell); }
 
 int launch_dt_partials_f32(const float* u, long plane, int pitch, int nx, int ny, double gamma, double* part,
                            int max_parts, hipStream_t s) {

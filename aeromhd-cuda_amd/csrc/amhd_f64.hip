@@ -7,9 +7,11 @@
 
 namespace amhd {
 
-hipError_t launch_stage_f64(int stage, bool powell, const StageParams<double>& a, hipStream_t s) {
-    return launch_stage_impl<double, ExactMath<double>>(stage, powell, a, s);
+hipError_t launch_stage_f64(int stage, bool powell, bool glm, const StageParams<double>& a, hipStream_t s) {
+    return launch_stage_impl<double, ExactMath<double>>(stage, powell, glm, a, s);
 }
+
+int stage_waves_per_cu_f64(bool powell) { return stage_waves_per_cu_impl<double, ExactMath<double>>(powell); }
 
 int launch_dt_partials_f64(const double* u, long plane, int pitch, int nx, int ny, double gamma, double* part,
                            int max_parts, hipStream_t s) {

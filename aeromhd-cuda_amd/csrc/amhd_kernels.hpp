@@ -42,17 +42,39 @@ template <class real> struct StageParams {
     const real* u0;   // U^n (interior layout)
     const real* us;   // stage input U^(s)
     real* uo;         // stage output (interior layout), or R (STAGE 0)
-    long plane;       // plane stride of u0/us/uo
+    long plane;       // plane stride of u0/us (elements)
     long oplane;      // plane stride of uo (R output may be ghosted)
     int pitch, opitch;
+    unsigned in_bytes, out_bytes;   // addressable bytes from u0/us and from uo
     int nx, ny, bcx_periodic, bcy_periodic;
-    int n_strips, n_segs, rows;
+    int n_strips, n_segs;   // wave w: strip w % n_strips, rows [seg*nx/n_segs, (seg+1)*nx/n_segs)
     double gamma, dx, dy;
     double dt, ch, decay;   // used when ctl == nullptr
     const StepCtl* ctl;     // device-resident dt/ch/decay
     double* part_dt;        // STAGE 3: [n_waves][4]
     double* part_bf;        // STAGE 1: [n_strips][2] (non-periodic x only)
 };
+
+// ---------------------------------------------------------- buffer access
+// Raw buffer loads/stores: one 32-bit per-lane offset (the column), the row
+// and variable plane go into the scalar offset, so a row of 10 variables
+// costs no vector address arithmetic.  Offsets past `bytes` read as 0.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* p, unsigned bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ float bload(float, __amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so) {
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, vo, so, 0));
+}
+typedef unsigned int u32x2 __attribute__((__vector_size__(8)));
+__device__ __forceinline__ double bload(double, __amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, vo, so, 0));
+}
+__device__ __forceinline__ void bstore(float x, __amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so) {
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(x), r, vo, so, 0);
+}
+__device__ __forceinline__ void bstore(double x, __amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, x), r, vo, so, 0);
+}
 
 // XCD-aware workgroup order: the hardware deals workgroups round-robin over
 // the 8 XCDs; renumber so that consecutive logical workgroups (neighbouring
@@ -75,7 +97,11 @@ __device__ __forceinline__ void load_cell(const real* __restrict__ base, long pl
     for (int v = 0; v < NV; ++v) u[v] = base[v * plane];
 }
 
-// FP64 CFL contributions of one cell (solver.py:209-214), from conserved u.
+// FP64 CFL contributions of one cell (solver.py:209-214), from conserved u:
+// m[0] collects cf^2 = gamma p/rho + B^2/rho (sqrt is monotone, so
+// sqrt(max cf^2) == max cf; the square root is taken once per step),
+// m[1] |vx|+|vy|, m[2] |vx|, m[3] |vy|.  FP64 state: the reference's
+// expression; FP32 state: (gamma p + B^2)/rho in FP64.
 template <class real>
 __device__ __forceinline__ void dt_terms(const real (&u)[NV], double gamma, double (&m)[NPART_DT]) {
     double ud[NV], w[NV];
@@ -83,9 +109,11 @@ __device__ __forceinline__ void dt_terms(const real (&u)[NV], double gamma, doub
     for (int v = 0; v < NV; ++v) ud[v] = double(u[v]);
     cons_to_prim<double, ExactMath<double>>(ud, w, gamma, false);
     const double b2 = w[iBX] * w[iBX] + w[iBY] * w[iBY] + w[iBZ] * w[iBZ];
-    const double cf = sqrt(gamma * w[iPR] / w[iRHO] + b2 / w[iRHO]);
+    double cf2;
+    if constexpr (sizeof(real) == sizeof(double)) cf2 = gamma * w[iPR] / w[iRHO] + b2 / w[iRHO];
+    else cf2 = (gamma * w[iPR] + b2) / w[iRHO];
     const double ax = fabs(w[iVX]), ay = fabs(w[iVY]);
-    m[0] = fmax(m[0], cf);
+    m[0] = fmax(m[0], cf2);
     m[1] = fmax(m[1], ax + ay);
     m[2] = fmax(m[2], ax);
     m[3] = fmax(m[3], ay);
@@ -110,13 +138,17 @@ __device__ __forceinline__ double bflux_term(const real (&u)[NV], double gamma, 
 // STAGE 2: U2 = 0.75 U0 + 0.25 (U1 + dt L(U1))             (solver.py:323)
 // STAGE 3: U  = (1/3) U0 + (2/3) (U2 + dt L(U2)), psi decay (solver.py:327-333)
 // followed by the scalar bounds of solver.py:199-204 for stages 1-3.
-template <class real, class P, int STAGE, bool POWELL>
+template <class real, class P, int STAGE, bool POWELL, bool GLM>
 __global__ __launch_bounds__(WAVE* WPB) void stage_kernel(StageParams<real> a) {
     const int lane = threadIdx.x & (WAVE - 1);
-    const int wave = xcd_logical_block() * WPB + (threadIdx.x >> 6);
+    // wave-uniform quantities live in scalar registers
+    const int wave = __builtin_amdgcn_readfirstlane(xcd_logical_block() * WPB + (int)(threadIdx.x >> 6));
     if (wave >= a.n_strips * a.n_segs) return;
     const int strip = wave % a.n_strips;
     const int seg = wave / a.n_strips;
+    const int nx = a.nx, ny = a.ny;
+    const int i0 = (int)(((long)seg * nx) / a.n_segs);
+    const int i1 = (int)(((long)(seg + 1) * nx) / a.n_segs);
 
     double dt_d = a.dt, ch_d = a.ch, decay_d = a.decay;
     if (a.ctl) {
@@ -128,74 +160,97 @@ __global__ __launch_bounds__(WAVE* WPB) void stage_kernel(StageParams<real> a) {
     const real dxr = real(a.dx), dyr = real(a.dy);
     const real idx = real(1.0 / a.dx), idy = real(1.0 / a.dy);
 
-    const int nx = a.nx, ny = a.ny;
-    const long plane = a.plane;
+    constexpr unsigned ES = sizeof(real);
+    const unsigned pb = (unsigned)a.plane * ES;             // plane stride, bytes
+    const unsigned rb = (unsigned)a.pitch * ES;             // row stride, bytes
     const int j = strip * STRIP - HALO + lane;
     const int sj = remap(j, ny, a.bcy_periodic);
+    const unsigned vo = (unsigned)sj * ES;
     const bool out_lane = lane >= HALO && lane < WAVE - HALO && j < ny;
-    const int i0 = seg * a.rows;
-    const int i1 = min(i0 + a.rows, nx);
-    const bool psi_x_ghost_zero = !a.bcx_periodic;
+    const __amdgpu_buffer_rsrc_t r_us = make_rsrc(a.us, a.in_bytes);
+    const __amdgpu_buffer_rsrc_t r_u0 = make_rsrc(a.u0, a.in_bytes);
+    const __amdgpu_buffer_rsrc_t r_uo = make_rsrc(a.uo, a.out_bytes);
+    const bool xper = a.bcx_periodic != 0;
 
-    auto row_ptr = [&](const real* base, int i) -> const real* {
-        return base + (long)remap(i, nx, a.bcx_periodic) * a.pitch + sj;
+    // x ghost rows: wrap (periodic) or clamp; psi = 0 on non-periodic x ghosts
+    auto row_src = [&](int i) -> int {
+        if (i < 0) return xper ? i + nx : 0;
+        if (i >= nx) return xper ? i - nx : nx - 1;
+        return i;
     };
-    auto ghost_row = [&](int i) -> bool { return psi_x_ghost_zero && (i < 0 || i >= nx); };
+    auto load_row = [&](__amdgpu_buffer_rsrc_t r, int si, real (&u)[NV]) {
+        const unsigned so = (unsigned)si * rb;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) u[v] = bload(real(0), r, vo, so + (unsigned)v * pb);
+    };
+    auto prim_row = [&](int i, real (&w)[NV]) {
+        real u[NV];
+        load_row(r_us, row_src(i), u);
+        cons_to_prim<real, P>(u, w, kc.gamma, !xper && (i < 0 || i >= nx));
+    };
 
-    real wm1[NV], w0[NV], wp1[NV];
-    real xl[NV], fxl[NV];
+    // prologue: W at rows i0, i0+1, the x-differences and the flux through
+    // face i0-1/2.  The row loop keeps three primitive rows (A, B, C) and
+    // three x-flux rows (F0, F1, F2) whose roles rotate with period 3, so
+    // the window slides without register copies.
+    real A[NV], B[NV], C[NV];
+    real F0[NV], F1[NV], F2[NV];
+    real xl[NV], dxp[NV];   // left state of face i+1/2; W_{i+1} - W_i
+    real bxm;               // Bx of row i-1 (Powell)
     {
-        real u[NV], wm2[NV], dm1[NV], d0[NV], fa[NV], fb[NV];
-        load_cell(row_ptr(a.us, i0 - 2), plane, u);
-        cons_to_prim<real, P>(u, wm2, kc.gamma, ghost_row(i0 - 2));
-        load_cell(row_ptr(a.us, i0 - 1), plane, u);
-        cons_to_prim<real, P>(u, wm1, kc.gamma, ghost_row(i0 - 1));
-        load_cell(row_ptr(a.us, i0), plane, u);
-        cons_to_prim<real, P>(u, w0, kc.gamma, ghost_row(i0));
-        load_cell(row_ptr(a.us, i0 + 1), plane, u);
-        cons_to_prim<real, P>(u, wp1, kc.gamma, ghost_row(i0 + 1));
+        real wm2[NV], wm1[NV], fa[NV], fb[NV];
+        prim_row(i0 - 2, wm2);
+        prim_row(i0 - 1, wm1);
+        prim_row(i0, A);
+        prim_row(i0 + 1, B);
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
-            dm1[v] = vl_slope<real, P>(wm2[v], wm1[v], w0[v]);
-            d0[v] = vl_slope<real, P>(wm1[v], w0[v], wp1[v]);
-            fa[v] = wm1[v] + real(0.5) * dm1[v];
-            fb[v] = w0[v] - real(0.5) * d0[v];
-            xl[v] = w0[v] + real(0.5) * d0[v];
+            const real dmm = wm1[v] - wm2[v], dm0 = A[v] - wm1[v];
+            dxp[v] = B[v] - A[v];
+            const real s_m1 = vl_slope_d<real, P>(dmm, dm0);
+            const real s_0 = vl_slope_d<real, P>(dm0, dxp[v]);
+            fa[v] = wm1[v] + real(0.5) * s_m1;
+            fb[v] = A[v] - real(0.5) * s_0;
+            xl[v] = A[v] + real(0.5) * s_0;
         }
         floor_face(fa); floor_face(fb); floor_face(xl);
-        hlld_face<real, P>(fa, fb, fxl, kc);
+        hlld_face<real, P, GLM>(fa, fb, F0, kc);
+        bxm = wm1[iBX];
     }
 
     double mx[NPART_DT] = {0.0, 0.0, 0.0, 0.0};
     double bf_sum[2] = {0.0, 0.0};
     real upre[NV];
-    load_cell(row_ptr(a.us, i0 + 2), plane, upre);
+    load_row(r_us, row_src(i0 + 2), upre);
 
-    for (int i = i0; i < i1; ++i) {
-        real wp2[NV];
-        cons_to_prim<real, P>(upre, wp2, kc.gamma, ghost_row(i + 2));
-        if (i + 1 < i1) load_cell(row_ptr(a.us, i + 3), plane, upre);
+    // One output row i.  w0 = W_i, wp1 = W_{i+1}; wp2 receives W_{i+2};
+    // fxl = F_{i-1/2}; fxr receives F_{i+1/2}.
+    auto row = [&](int i, real (&w0)[NV], real (&wp1)[NV], real (&wp2)[NV], const real (&fxl)[NV],
+                   real (&fxr)[NV]) __attribute__((always_inline)) {
+        cons_to_prim<real, P>(upre, wp2, kc.gamma, !xper && (i + 2 >= nx));
+        if (i + 1 < i1) load_row(r_us, row_src(i + 3), upre);
 
-        // stage input and base at this row (interior; halo lanes read a
-        // valid remapped address and never store)
+        // stage input and base at this row (halo lanes read a valid
+        // remapped column and never store)
         real ucur[NV], ubase[NV];
-        const long off = (long)i * a.pitch + sj;
-        load_cell(a.us + off, plane, ucur);
-        if constexpr (STAGE >= 2) load_cell(a.u0 + off, plane, ubase);
+        load_row(r_us, i, ucur);
+        if constexpr (STAGE >= 2) load_row(r_u0, i, ubase);
 
-        // ---- x face i+1/2
-        real fxr[NV];
+        // ---- x face i+1/2: slope at row i+1 from the differences
         {
-            real xr[NV], d1[NV];
+            real xr[NV], xn[NV];
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
-                d1[v] = vl_slope<real, P>(w0[v], wp1[v], wp2[v]);
-                xr[v] = wp1[v] - real(0.5) * d1[v];
+                const real dn = wp2[v] - wp1[v];
+                const real d1 = vl_slope_d<real, P>(dxp[v], dn);
+                dxp[v] = dn;
+                xr[v] = wp1[v] - real(0.5) * d1;
+                xn[v] = wp1[v] + real(0.5) * d1;
             }
             floor_face(xr);
-            hlld_face<real, P>(xl, xr, fxr, kc);
+            hlld_face<real, P, GLM>(xl, xr, fxr, kc);
 #pragma unroll
-            for (int v = 0; v < NV; ++v) xl[v] = wp1[v] + real(0.5) * d1[v];
+            for (int v = 0; v < NV; ++v) xl[v] = xn[v];
             floor_face(xl);
         }
 
@@ -206,15 +261,16 @@ __global__ __launch_bounds__(WAVE* WPB) void stage_kernel(StageParams<real> a) {
             real yl[NV], yr[NV];
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
-                const real wpv = from_prev(w0[v]);
                 const real wnv = from_next(w0[v]);
-                if (v == iBY) { by_prev = wpv; by_next = wnv; }
-                const real d = vl_slope<real, P>(wpv, w0[v], wnv);
+                const real dp = wnv - w0[v];
+                const real dm = from_prev(dp);
+                if (v == iBY) { by_next = wnv; by_prev = from_prev(w0[v]); }
+                const real d = vl_slope_d<real, P>(dm, dp);
                 yl[v] = w0[v] + real(0.5) * d;
                 yr[v] = from_next(w0[v] - real(0.5) * d);
             }
             floor_face(yl); floor_face(yr);
-            hlld_face_y<real, P>(yl, yr, fy, kc);
+            hlld_face_y<real, P, GLM>(yl, yr, fy, kc);
         }
 
         // ---- RHS: ((0 - dFx/dx) - dFy/dy) + S   (solver.py:297-306)
@@ -222,9 +278,10 @@ __global__ __launch_bounds__(WAVE* WPB) void stage_kernel(StageParams<real> a) {
         if constexpr (POWELL) {
             // solver.py:249-254, on the BC-filled primitive field
             const real half = real(0.5);
-            divb = P::divi(half * (w0[iBX] + wp1[iBX]) - half * (wm1[iBX] + w0[iBX]), dxr, idx) +
+            divb = P::divi(half * (w0[iBX] + wp1[iBX]) - half * (bxm + w0[iBX]), dxr, idx) +
                    P::divi(half * (w0[iBY] + by_next) - half * (by_prev + w0[iBY]), dyr, idy);
         }
+        bxm = w0[iBX];
         real unew[NV];
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
@@ -261,28 +318,33 @@ __global__ __launch_bounds__(WAVE* WPB) void stage_kernel(StageParams<real> a) {
 
         if (out_lane) {
             if constexpr (STAGE == 0) {
-                real* o = a.uo + (long)i * a.opitch + j;
+                const unsigned so = (unsigned)i * (unsigned)a.opitch * ES;
+                const unsigned ob = (unsigned)a.oplane * ES;
 #pragma unroll
-                for (int v = 0; v < NV; ++v) o[v * a.oplane] = unew[v];
+                for (int v = 0; v < NV; ++v) bstore(unew[v], r_uo, vo, so + (unsigned)v * ob);
             } else {
-                real* o = a.uo + off;
+                const unsigned so = (unsigned)i * rb;
 #pragma unroll
-                for (int v = 0; v < NV; ++v) o[v * plane] = unew[v];
+                for (int v = 0; v < NV; ++v) bstore(unew[v], r_uo, vo, so + (unsigned)v * pb);
             }
             if constexpr (STAGE == 3) dt_terms(unew, a.gamma, mx);
             if constexpr (STAGE == 1) {
-                if (!a.bcx_periodic) {
+                if (!xper) {
                     if (i == 0) bf_sum[0] += bflux_term(ucur, a.gamma, a.dy);
                     if (i == nx - 1) bf_sum[1] += bflux_term(ucur, a.gamma, a.dy);
                 }
             }
         }
+    };
 
-        // slide the window
-#pragma unroll
-        for (int v = 0; v < NV; ++v) {
-            wm1[v] = w0[v]; w0[v] = wp1[v]; wp1[v] = wp2[v]; fxl[v] = fxr[v];
-        }
+    for (int i = i0;;) {
+        if (i >= i1) break;
+        row(i, A, B, C, F0, F1);
+        if (++i >= i1) break;
+        row(i, B, C, A, F1, F2);
+        if (++i >= i1) break;
+        row(i, C, A, B, F2, F0);
+        ++i;
     }
 
     if constexpr (STAGE == 3) {
@@ -297,7 +359,7 @@ __global__ __launch_bounds__(WAVE* WPB) void stage_kernel(StageParams<real> a) {
         }
     }
     if constexpr (STAGE == 1) {
-        if (a.part_bf && !a.bcx_periodic) {
+        if (a.part_bf && !xper) {
             const double s0 = wave_sum(bf_sum[0]);
             const double s1 = wave_sum(bf_sum[1]);
             if (lane == 0) {
@@ -351,9 +413,9 @@ __global__ __launch_bounds__(256) void dt_partials_w_kernel(const double* __rest
         const double p = fmax(W[iPR * n + k], FLOOR_PR);
         const double bx = W[iBX * n + k], by = W[iBY * n + k], bz = W[iBZ * n + k];
         const double b2 = bx * bx + by * by + bz * bz;
-        const double cf = sqrt(gamma * p / rho + b2 / rho);
+        const double cf2 = gamma * p / rho + b2 / rho;
         const double ax = fabs(W[iVX * n + k]), ay = fabs(W[iVY * n + k]);
-        m[0] = fmax(m[0], cf);
+        m[0] = fmax(m[0], cf2);
         m[1] = fmax(m[1], ax + ay);
         m[2] = fmax(m[2], ax);
         m[3] = fmax(m[3], ay);
@@ -426,11 +488,12 @@ __global__ __launch_bounds__(256) void step_ctl_kernel(StepCtl* ctl, const doubl
     }
     if (mode == CTL_DEVICE_LOOP && ctl->done) return;
 
-    const double ch = fmax(r[0] + r[1], 1.0) * 1.5;
+    const double cfmax = sqrt(r[0]);   // partials hold max cf^2
+    const double ch = fmax(cfmax + r[1], 1.0) * 1.5;
     const double sx = r[2] + ch, sy = r[3] + ch;
     const double sm = fmax(sx / dx, sy / dy);
     double dt = (sm < 1e-14) ? cfl * fmin(dx, dy) : cfl / sm;
-    ctl->cfmax = r[0];
+    ctl->cfmax = cfmax;
     ctl->vmax = r[1];
     ctl->ch = ch;
     if (mode == CTL_DT_ONLY) {

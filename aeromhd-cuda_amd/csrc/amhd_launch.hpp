@@ -10,8 +10,12 @@
 namespace amhd {
 
 // One SSP-RK3 stage (0 = RHS only) of the fused strip-march kernel.
-hipError_t launch_stage_f64(int stage, bool powell, const StageParams<double>& a, hipStream_t s);
-hipError_t launch_stage_f32(int stage, bool powell, const StageParams<float>& a, hipStream_t s);
+hipError_t launch_stage_f64(int stage, bool powell, bool glm, const StageParams<double>& a, hipStream_t s);
+hipError_t launch_stage_f32(int stage, bool powell, bool glm, const StageParams<float>& a, hipStream_t s);
+
+// Resident wavefronts per CU of the stage kernel (occupancy query).
+int stage_waves_per_cu_f64(bool powell);
+int stage_waves_per_cu_f32(bool powell);
 
 // CFL partial maxima over an interior-layout state; returns #partials.
 int launch_dt_partials_f64(const double* u, long plane, int pitch, int nx, int ny, double gamma, double* part,
@@ -37,22 +41,39 @@ inline int grid_for(long n, int block, int cap = 4096) {
 
 // Template bodies, included by the two precision TUs only.
 #ifdef AMHD_LAUNCH_IMPL
+template <class real, class P, int STAGE>
+void launch_stage_t(bool powell, bool glm, const StageParams<real>& a, dim3 grid, dim3 block, hipStream_t s) {
+    if (powell) {
+        if (glm) stage_kernel<real, P, STAGE, true, true><<<grid, block, 0, s>>>(a);
+        else stage_kernel<real, P, STAGE, true, false><<<grid, block, 0, s>>>(a);
+    } else {
+        if (glm) stage_kernel<real, P, STAGE, false, true><<<grid, block, 0, s>>>(a);
+        else stage_kernel<real, P, STAGE, false, false><<<grid, block, 0, s>>>(a);
+    }
+}
+
+// glm: the caller guarantees ch > 0 for this launch (the device loop always
+// has ch >= 1.5; host-driven steps pass their ch).
 template <class real, class P>
-hipError_t launch_stage_impl(int stage, bool powell, const StageParams<real>& a, hipStream_t s) {
+hipError_t launch_stage_impl(int stage, bool powell, bool glm, const StageParams<real>& a, hipStream_t s) {
     const int n_waves = a.n_strips * a.n_segs;
     const dim3 grid((n_waves + WPB - 1) / WPB), block(WAVE * WPB);
-    switch (stage * 2 + (powell ? 1 : 0)) {
-        case 0: stage_kernel<real, P, 0, false><<<grid, block, 0, s>>>(a); break;
-        case 1: stage_kernel<real, P, 0, true><<<grid, block, 0, s>>>(a); break;
-        case 2: stage_kernel<real, P, 1, false><<<grid, block, 0, s>>>(a); break;
-        case 3: stage_kernel<real, P, 1, true><<<grid, block, 0, s>>>(a); break;
-        case 4: stage_kernel<real, P, 2, false><<<grid, block, 0, s>>>(a); break;
-        case 5: stage_kernel<real, P, 2, true><<<grid, block, 0, s>>>(a); break;
-        case 6: stage_kernel<real, P, 3, false><<<grid, block, 0, s>>>(a); break;
-        case 7: stage_kernel<real, P, 3, true><<<grid, block, 0, s>>>(a); break;
+    switch (stage) {
+        case 0: launch_stage_t<real, P, 0>(powell, glm, a, grid, block, s); break;
+        case 1: launch_stage_t<real, P, 1>(powell, glm, a, grid, block, s); break;
+        case 2: launch_stage_t<real, P, 2>(powell, glm, a, grid, block, s); break;
+        case 3: launch_stage_t<real, P, 3>(powell, glm, a, grid, block, s); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+template <class real, class P>
+int stage_waves_per_cu_impl(bool powell) {
+    int nb = 0;
+    hipError_t e = powell ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, stage_kernel<real, P, 2, true, true>, WAVE * WPB, 0)
+                          : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, stage_kernel<real, P, 2, false, true>, WAVE * WPB, 0);
+    return e == hipSuccess && nb > 0 ? nb * WPB : WPB;
 }
 
 template <class real>

@@ -297,6 +297,13 @@ class MHDSolver:
         return self.step - s0
 
     # ------------------------------------------------------------ analysis
+    def global_sums(self):
+        """FP64 device sums of the current state: (mass, energy, max|divB|,
+        L2 divB) with mass = sum(rho) dx dy, energy as solver.py:430 and
+        divB as solver.py:419-428."""
+        self._sync_state()
+        return self.ctx.global_sums()
+
     def compute_diagnostics(self):
         """Diagnostics of solver.py:335-444 on the downloaded state."""
         from amhd_diag import diagnostics

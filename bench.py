@@ -79,7 +79,7 @@ def gpu_measure(name: str, steps: int, warmup: int, seed: int, rows: int | None)
     esize = 4 if cfg.precision == "fp32" else 8
     bytes_step = 8 * NVAR * esize * cells
     kern_s = sum(stage_ms) * 1e-3 / steps
-    sums = s.ctx.global_sums()
+    sums = s.global_sums()
     return {
         "desc": desc, "cfg": cfg, "cells": cells, "esize": esize, "rows": s.ctx.rows,
         "mcups": cells * steps / wall / 1e6, "ms_per_step": wall * 1e3 / steps,

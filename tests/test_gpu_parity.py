@@ -243,9 +243,9 @@ def test_full_size_conservation(torch_gpu, precision, nx, ny):
     U = prim_to_cons_np(smooth_state(nx, ny, seed=1), cfg.gamma)
     s = MHDSolver(cfg)
     s.U = U
-    m0, e0, _, _ = s.ctx.global_sums()
+    m0, e0, _, _ = s.global_sums()
     assert s.advance(20) == 20
-    m1, e1, divmax, _ = s.ctx.global_sums()
+    m1, e1, divmax, _ = s.global_sums()
     tol = 1e-13 if precision == "fp64" else 2e-6
     assert abs(m1 - m0) <= tol * abs(m0) and abs(e1 - e0) <= tol * abs(e0), (m0, m1, e0, e1)
     assert np.isfinite(divmax)
