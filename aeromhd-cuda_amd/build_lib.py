@@ -4,7 +4,8 @@ Three translation units, compiled by hipcc in-tree:
 
 * ``csrc/amhd_f64.hip``  FP64 fused update, ``-ffp-contract=off`` (IEEE
   per-operation arithmetic, as the reference's ufuncs);
-* ``csrc/amhd_f32.hip``  FP32 fused update, ``-ffp-contract=fast``;
+* ``csrc/amhd_f32.hip``  FP32 fused update, ``-ffp-contract=on`` (multiply-adds fused per
+  source expression, so a body inlined into different kernels rounds the same);
 * ``csrc/amhd_api.hip``  the C ABI of ``include/aeromhd.h`` and the FP64
   standalone operators, ``-ffp-contract=off``.
 
@@ -33,7 +34,7 @@ COMMON = [
 ]
 UNITS = [
     ("amhd_f64.hip", ["-ffp-contract=off"]),
-    ("amhd_f32.hip", ["-ffp-contract=fast"]),
+    ("amhd_f32.hip", ["-ffp-contract=on"]),
     ("amhd_api.hip", ["-ffp-contract=off"]),
 ]
 

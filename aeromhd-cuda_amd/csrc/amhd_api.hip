@@ -659,3 +659,261 @@ int amhd_compute_dt_w(const double* W, long n, double gamma, double cfl, double 
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------ batch
+namespace {
+
+struct Batch {
+    int n = 0, dtype = AMHD_F32;
+    std::vector<int> nx, ny;
+    std::vector<amhd_params> p;
+    std::vector<BatchInst> inst;       // host copy of the device descriptors
+    std::vector<int> n_part;           // CFL partial count when they come from the standalone kernel
+    std::vector<char> part_valid, part_from_stage3;
+    std::vector<int> part_cap;
+    BatchInst* dinst = nullptr;
+    StepCtl* ctl = nullptr;            // [n]
+    double* staging = nullptr;
+    size_t staging_bytes = 0;
+    int rows = 0, n_tasks = 0, resident = 0;
+    hipStream_t stream = nullptr;
+    hipGraphExec_t graph = nullptr;
+};
+
+int batch_layout(Batch* b, int rows) {
+    b->rows = rows;
+    long total = 0;
+    for (int k = 0; k < b->n; ++k) total += (long)b->nx[k] * ((b->ny[k] + STRIP - 1) / STRIP);
+    int r = rows > 0 ? rows : (int)std::max<long>(1, (total + b->resident - 1) / b->resident);
+    int base = 0;
+    for (int k = 0; k < b->n; ++k) {
+        BatchInst& d = b->inst[k];
+        d.n_strips = (b->ny[k] + STRIP - 1) / STRIP;
+        d.n_segs = std::max(1, std::min((b->nx[k] + r - 1) / r, b->nx[k]));
+        d.task_base = base;
+        base += d.n_strips * d.n_segs;
+    }
+    b->n_tasks = base;
+    HIP_TRY(hipMemcpyAsync(b->dinst, b->inst.data(), sizeof(BatchInst) * b->n, hipMemcpyHostToDevice, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    if (b->graph) { hipGraphExecDestroy(b->graph); b->graph = nullptr; }
+    for (int k = 0; k < b->n; ++k) b->part_from_stage3[k] = 0, b->part_valid[k] = 0;
+    return 0;
+}
+
+int batch_read_ctl(Batch* b, std::vector<StepCtl>& h) {
+    h.resize(b->n);
+    HIP_TRY(hipMemcpyAsync(h.data(), b->ctl, sizeof(StepCtl) * b->n, hipMemcpyDeviceToHost, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    return 0;
+}
+
+hipError_t batch_enqueue_step(Batch* b) {
+    hipError_t e = hipSuccess;
+    for (int st = 1; st <= 3 && e == hipSuccess; ++st)
+        e = b->dtype == AMHD_F64 ? launch_batch_stage_f64(st, b->dinst, b->n, b->n_tasks, b->resident, b->stream)
+                                 : launch_batch_stage_f32(st, b->dinst, b->n, b->n_tasks, b->resident, b->stream);
+    if (e != hipSuccess) return e;
+    batch_ctl_kernel<<<b->n, 256, 0, b->stream>>>(b->dinst, 0);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+extern "C" {
+
+int amhd_batch_destroy(void* handle) {
+    if (!handle) return 0;
+    Batch* b = (Batch*)handle;
+    if (b->stream) hipStreamSynchronize(b->stream);
+    if (b->graph) hipGraphExecDestroy(b->graph);
+    for (auto& d : b->inst) {
+        for (auto& x : d.buf) if (x) hipFree(x);
+        if (d.part_dt) hipFree(d.part_dt);
+        if (d.part_bf) hipFree(d.part_bf);
+    }
+    if (b->dinst) hipFree(b->dinst);
+    if (b->ctl) hipFree(b->ctl);
+    if (b->staging) hipFree(b->staging);
+    if (b->stream) hipStreamDestroy(b->stream);
+    delete b;
+    return 0;
+}
+
+int amhd_batch_create(void** out, int n, const int* nx, const int* ny, int dtype, const amhd_params* params) {
+    if (!out || !nx || !ny || !params || n <= 0) return fail(-1, "amhd_batch_create: bad argument");
+    *out = nullptr;
+    if (dtype != AMHD_F64 && dtype != AMHD_F32) return fail(-1, "amhd_batch_create: unknown dtype %d", dtype);
+    Batch* b = new Batch();
+    b->n = n; b->dtype = dtype;
+    b->nx.assign(nx, nx + n); b->ny.assign(ny, ny + n); b->p.assign(params, params + n);
+    b->inst.resize(n); b->n_part.assign(n, 0); b->part_valid.assign(n, 0); b->part_from_stage3.assign(n, 0);
+    b->part_cap.assign(n, 0);
+    hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    int dev = 0, cus = 0;
+    if (e == hipSuccess) e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    b->resident = std::max(1, cus) * (dtype == AMHD_F64 ? batch_waves_per_cu_f64() : batch_waves_per_cu_f32());
+    for (int k = 0; k < n && e == hipSuccess; ++k) {
+        const amhd_params& p = params[k];
+        if (nx[k] < 2 || ny[k] < 2 || !(p.gamma > 1.0) || !(p.dx > 0.0) || !(p.dy > 0.0)) {
+            amhd_batch_destroy(b);
+            return fail(-1, "amhd_batch_create: instance %d: need nx, ny >= 2, gamma > 1, dx, dy > 0", k);
+        }
+        BatchInst& d = b->inst[k];
+        std::memset(&d, 0, sizeof d);
+        d.nx = nx[k]; d.ny = ny[k]; d.pitch = ny[k]; d.plane = (long)nx[k] * ny[k];
+        const double bytes = (double)esize(dtype) * NV * (double)d.plane;
+        if (bytes > 4294967295.0) {
+            amhd_batch_destroy(b);
+            return fail(-4, "amhd_batch_create: instance %d exceeds 4 GiB", k);
+        }
+        d.bytes = (unsigned)bytes;
+        d.bcx_periodic = p.bc_x_periodic; d.bcy_periodic = p.bc_y_periodic; d.powell = p.powell;
+        d.gamma = p.gamma; d.dx = p.dx; d.dy = p.dy;
+        const int nstr = (ny[k] + STRIP - 1) / STRIP;
+        b->part_cap[k] = std::max(nx[k] * nstr, SCRATCH_PARTS);
+        for (int q = 0; q < 3 && e == hipSuccess; ++q) e = hipMalloc(&d.buf[q], (size_t)bytes);
+        if (e == hipSuccess) e = hipMalloc(&d.part_dt, sizeof(double) * NPART_DT * (size_t)b->part_cap[k]);
+        if (e == hipSuccess) e = hipMalloc(&d.part_bf, sizeof(double) * 2 * (size_t)nstr);
+        if (e == hipSuccess) e = hipMemsetAsync(d.part_bf, 0, sizeof(double) * 2 * (size_t)nstr, b->stream);
+        b->staging_bytes = std::max(b->staging_bytes, sizeof(double) * NV * (size_t)(nx[k] + 4) * (ny[k] + 4));
+    }
+    if (e == hipSuccess) e = hipMalloc(&b->staging, b->staging_bytes);
+    if (e == hipSuccess) e = hipMalloc(&b->ctl, sizeof(StepCtl) * n);
+    if (e == hipSuccess) e = hipMalloc(&b->dinst, sizeof(BatchInst) * n);
+    if (e == hipSuccess) {
+        std::vector<StepCtl> z(n);
+        std::memset(z.data(), 0, sizeof(StepCtl) * n);
+        for (int k = 0; k < n; ++k) b->inst[k].ctl = b->ctl + k;
+        e = hipMemcpyAsync(b->ctl, z.data(), sizeof(StepCtl) * n, hipMemcpyHostToDevice, b->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    }
+    if (e != hipSuccess) {
+        int rc = fail(e == hipErrorOutOfMemory ? -3 : -2, "amhd_batch_create: %s", hipGetErrorString(e));
+        amhd_batch_destroy(b);
+        return rc;
+    }
+    int rc = batch_layout(b, 0);
+    if (rc) { amhd_batch_destroy(b); return rc; }
+    *out = b;
+    return 0;
+}
+
+int amhd_batch_set_rows(void* handle, int rows) {
+    if (!handle || rows < 0) return fail(-1, "amhd_batch_set_rows: bad argument");
+    Batch* b = (Batch*)handle;
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    return batch_layout(b, rows);
+}
+
+int amhd_batch_info(void* handle, int* n_tasks, int* resident, int* rows) {
+    if (!handle) return fail(-1, "amhd_batch_info: null handle");
+    Batch* b = (Batch*)handle;
+    if (n_tasks) *n_tasks = b->n_tasks;
+    if (resident) *resident = b->resident;
+    if (rows) {
+        int r = 0;
+        for (int k = 0; k < b->n; ++k) r = std::max(r, (b->nx[k] + b->inst[k].n_segs - 1) / b->inst[k].n_segs);
+        *rows = r;
+    }
+    return 0;
+}
+
+int amhd_batch_upload(void* handle, int k, const double* U, int on_device) {
+    if (!handle || !U) return fail(-1, "amhd_batch_upload: null argument");
+    Batch* b = (Batch*)handle;
+    if (k < 0 || k >= b->n) return fail(-1, "amhd_batch_upload: instance %d out of range", k);
+    std::vector<StepCtl> h;
+    int rc = batch_read_ctl(b, h);
+    if (rc) return rc;
+    const BatchInst& d = b->inst[k];
+    const size_t gb = sizeof(double) * NV * (size_t)(b->nx[k] + 4) * (b->ny[k] + 4);
+    HIP_TRY(hipMemcpyAsync(b->staging, U, gb, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, b->stream));
+    HIP_TRY(b->dtype == AMHD_F64 ? launch_ingest_f64(b->staging, (double*)d.buf[h[k].cur], d.plane, d.pitch, d.nx, d.ny, b->stream)
+                                 : launch_ingest_f32(b->staging, (float*)d.buf[h[k].cur], d.plane, d.pitch, d.nx, d.ny, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    b->part_valid[k] = 0;
+    b->part_from_stage3[k] = 0;
+    return 0;
+}
+
+int amhd_batch_download(void* handle, int k, double* U, int on_device) {
+    if (!handle || !U) return fail(-1, "amhd_batch_download: null argument");
+    Batch* b = (Batch*)handle;
+    if (k < 0 || k >= b->n) return fail(-1, "amhd_batch_download: instance %d out of range", k);
+    std::vector<StepCtl> h;
+    int rc = batch_read_ctl(b, h);
+    if (rc) return rc;
+    const BatchInst& d = b->inst[k];
+    HIP_TRY(b->dtype == AMHD_F64
+                ? launch_materialize_f64((const double*)d.buf[h[k].cur], d.plane, d.pitch, d.nx, d.ny, d.bcx_periodic,
+                                         d.bcy_periodic, b->staging, b->stream)
+                : launch_materialize_f32((const float*)d.buf[h[k].cur], d.plane, d.pitch, d.nx, d.ny, d.bcx_periodic,
+                                         d.bcy_periodic, b->staging, b->stream));
+    const size_t gb = sizeof(double) * NV * (size_t)(b->nx[k] + 4) * (b->ny[k] + 4);
+    HIP_TRY(hipMemcpyAsync(U, b->staging, gb, on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    return 0;
+}
+
+int amhd_batch_run(void* handle, int n_steps, const double* cfl, const double* t_end, const int* max_steps,
+                   int use_graph, double* t_inout, int* step_inout) {
+    if (!handle || !cfl || !t_end || !max_steps || !t_inout || !step_inout || n_steps < 0)
+        return fail(-1, "amhd_batch_run: bad argument");
+    Batch* b = (Batch*)handle;
+    std::vector<StepCtl> h;
+    int rc = batch_read_ctl(b, h);
+    if (rc) return rc;
+    std::vector<int> start(b->n);
+    for (int k = 0; k < b->n; ++k) {
+        const BatchInst& d = b->inst[k];
+        start[k] = step_inout[k];
+        h[k].t = t_inout[k]; h[k].steps = step_inout[k];
+        h[k].t_end = t_end[k]; h[k].cfl = cfl[k];
+        h[k].max_steps = (int)std::min<long>((long)max_steps[k], (long)step_inout[k] + n_steps);
+        h[k].glm_alpha = b->p[k].glm_alpha;
+        h[k].min_dxdy = std::min(b->p[k].dx, b->p[k].dy);
+        h[k].done = 0; h[k].pending = 0;
+        if (!b->part_valid[k]) {
+            const int g = b->dtype == AMHD_F64
+                              ? launch_dt_partials_f64((const double*)d.buf[h[k].cur], d.plane, d.pitch, d.nx, d.ny,
+                                                       d.gamma, d.part_dt, b->part_cap[k], b->stream)
+                              : launch_dt_partials_f32((const float*)d.buf[h[k].cur], d.plane, d.pitch, d.nx, d.ny,
+                                                       d.gamma, d.part_dt, b->part_cap[k], b->stream);
+            if (g < 0) return fail(-2, "amhd_batch_run: CFL partials launch failed");
+            b->n_part[k] = g;
+            b->part_valid[k] = 1;
+            b->part_from_stage3[k] = 0;
+        }
+        h[k].n_part = b->part_from_stage3[k] ? d.n_strips * d.n_segs : b->n_part[k];
+    }
+    HIP_TRY(hipMemcpyAsync(b->ctl, h.data(), sizeof(StepCtl) * b->n, hipMemcpyHostToDevice, b->stream));
+    batch_ctl_kernel<<<b->n, 256, 0, b->stream>>>(b->dinst, 1);
+    HIP_TRY(hipGetLastError());
+    if (use_graph && n_steps > 0) {
+        if (!b->graph) {
+            hipGraph_t g;
+            HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+            hipError_t e = batch_enqueue_step(b);
+            hipError_t e2 = hipStreamEndCapture(b->stream, &g);
+            HIP_TRY(e);
+            HIP_TRY(e2);
+            HIP_TRY(hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0));
+            hipGraphDestroy(g);
+        }
+        for (int s = 0; s < n_steps; ++s) HIP_TRY(hipGraphLaunch(b->graph, b->stream));
+    } else {
+        for (int s = 0; s < n_steps; ++s) HIP_TRY(batch_enqueue_step(b));
+    }
+    rc = batch_read_ctl(b, h);
+    if (rc) return rc;
+    for (int k = 0; k < b->n; ++k) {
+        if (h[k].steps > start[k]) b->part_from_stage3[k] = 1;
+        t_inout[k] = h[k].t;
+        step_inout[k] = h[k].steps;
+    }
+    return 0;
+}
+
+}  // extern "C"

@@ -1,5 +1,7 @@
 // amhd_f32.hip -- FP32 instantiation of the fused update (FastMath policy).
-// Built with -ffp-contract=fast: FP32 results are not bit-matched to the
+// Built with -ffp-contract=on (a*b+c fused per source expression, so the
+// same body gives the same bits in every kernel it is inlined into).  FP32
+// results are not bit-matched to the
 // reference; their error is measured against the FP64 oracle.
 #define AMHD_LAUNCH_IMPL
 #include "amhd_launch.hpp"
@@ -9,6 +11,13 @@ namespace amhd {
 hipError_t launch_stage_f32(int stage, bool powell, bool glm, const StageParams<float>& a, hipStream_t s) {
     return launch_stage_impl<float, FastMath<float>>(stage, powell, glm, a, s);
 }
+
+hipError_t launch_batch_stage_f32(int stage, const BatchInst* insts, int n_inst, int n_tasks, int resident,
+                                  hipStream_t s) {
+    return launch_batch_stage_impl<float, FastMath<float>>(stage, insts, n_inst, n_tasks, resident, s);
+}
+
+int batch_waves_per_cu_f32() { return batch_waves_per_cu_impl<float, FastMath<float>>(); }
 
 int stage_waves_per_cu_f32(bool powell) { return stage_waves_per_cu_impl<float, FastMath<float>>(powell); }
 

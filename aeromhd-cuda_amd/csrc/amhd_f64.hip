@@ -11,6 +11,13 @@ hipError_t launch_stage_f64(int stage, bool powell, bool glm, const StageParams<
     return launch_stage_impl<double, ExactMath<double>>(stage, powell, glm, a, s);
 }
 
+hipError_t launch_batch_stage_f64(int stage, const BatchInst* insts, int n_inst, int n_tasks, int resident,
+                                  hipStream_t s) {
+    return launch_batch_stage_impl<double, ExactMath<double>>(stage, insts, n_inst, n_tasks, resident, s);
+}
+
+int batch_waves_per_cu_f64() { return batch_waves_per_cu_impl<double, ExactMath<double>>(); }
+
 int stage_waves_per_cu_f64(bool powell) { return stage_waves_per_cu_impl<double, ExactMath<double>>(powell); }
 
 int launch_dt_partials_f64(const double* u, long plane, int pitch, int nx, int ny, double gamma, double* part,

@@ -36,6 +36,7 @@ struct StepCtl {
     double bflux, bflux_cum;
     double cfmax, vmax;
     int steps, max_steps, done, pending;
+    int cur, n_part;   // batched solver: current state buffer, CFL partial count
 };
 
 template <class real> struct StageParams {
@@ -138,12 +139,10 @@ __device__ __forceinline__ double bflux_term(const real (&u)[NV], double gamma, 
 // STAGE 2: U2 = 0.75 U0 + 0.25 (U1 + dt L(U1))             (solver.py:323)
 // STAGE 3: U  = (1/3) U0 + (2/3) (U2 + dt L(U2)), psi decay (solver.py:327-333)
 // followed by the scalar bounds of solver.py:199-204 for stages 1-3.
+// The work of one wavefront: strip `wave % n_strips`, segment
+// `wave / n_strips` of the grid described by `a`.  `wave` is wave-uniform.
 template <class real, class P, int STAGE, bool POWELL, bool GLM>
-__global__ __launch_bounds__(WAVE* WPB) void stage_kernel(StageParams<real> a) {
-    const int lane = threadIdx.x & (WAVE - 1);
-    // wave-uniform quantities live in scalar registers
-    const int wave = __builtin_amdgcn_readfirstlane(xcd_logical_block() * WPB + (int)(threadIdx.x >> 6));
-    if (wave >= a.n_strips * a.n_segs) return;
+__device__ __forceinline__ void stage_wave(const StageParams<real>& a, const int wave, const int lane) {
     const int strip = wave % a.n_strips;
     const int seg = wave / a.n_strips;
     const int nx = a.nx, ny = a.ny;
@@ -370,6 +369,15 @@ __global__ __launch_bounds__(WAVE* WPB) void stage_kernel(StageParams<real> a) {
     }
 }
 
+template <class real, class P, int STAGE, bool POWELL, bool GLM>
+__global__ __launch_bounds__(WAVE* WPB) void stage_kernel(StageParams<real> a) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    // wave-uniform quantities live in scalar registers
+    const int wave = __builtin_amdgcn_readfirstlane(xcd_logical_block() * WPB + (int)(threadIdx.x >> 6));
+    if (wave >= a.n_strips * a.n_segs) return;
+    stage_wave<real, P, STAGE, POWELL, GLM>(a, wave, lane);
+}
+
 // ------------------------------------------------- CFL partials from state
 // One partial per workgroup over the interior layout (used when no stage-3
 // partials are current, e.g. right after a state upload).
@@ -441,9 +449,9 @@ __global__ __launch_bounds__(256) void dt_partials_w_kernel(const double* __rest
 // cell and max_k (|v_k| + ch)/dx == (max_k |v_k| + ch)/dx, so the four maxima
 // determine dt bit for bit).  Also closes a step: boundary-flux sum and its
 // accumulation (solver.py:316-317), t and step advance (solver.py:481-482).
-__global__ __launch_bounds__(256) void step_ctl_kernel(StepCtl* ctl, const double* part_dt, int n_part,
-                                                       const double* part_bf, int n_bf, double dx, double dy,
-                                                       double host_dt, double cfl, int mode) {
+// One workgroup of 256 threads.  CTL_DEVICE_LOOP reads cfl from ctl.
+__device__ __forceinline__ void step_ctl_block(StepCtl* ctl, const double* part_dt, int n_part, const double* part_bf,
+                                               int n_bf, double dx, double dy, double host_dt, double cfl, int mode) {
     __shared__ double sh[NPART_DT][4];
     __shared__ double shb[2][4];
     double m[NPART_DT] = {0.0, 0.0, 0.0, 0.0};
@@ -484,9 +492,13 @@ __global__ __launch_bounds__(256) void step_ctl_kernel(StepCtl* ctl, const doubl
         ctl->bflux_cum += bflux * ctl->dt;
         ctl->t += ctl->dt;
         ctl->steps += 1;
+        ctl->cur = (ctl->cur + 1) % 3;
         ctl->pending = 0;
     }
-    if (mode == CTL_DEVICE_LOOP && ctl->done) return;
+    if (mode == CTL_DEVICE_LOOP) {
+        if (ctl->done) return;
+        cfl = ctl->cfl;
+    }
 
     const double cfmax = sqrt(r[0]);   // partials hold max cf^2
     const double ch = fmax(cfmax + r[1], 1.0) * 1.5;
@@ -508,6 +520,12 @@ __global__ __launch_bounds__(256) void step_ctl_kernel(StepCtl* ctl, const doubl
     ctl->dt = dt;
     ctl->decay = ch > 0.0 ? exp(-ctl->glm_alpha * ch * dt / ctl->min_dxdy) : 1.0;
     ctl->pending = 1;
+}
+
+__global__ __launch_bounds__(256) void step_ctl_kernel(StepCtl* ctl, const double* part_dt, int n_part,
+                                                       const double* part_bf, int n_bf, double dx, double dy,
+                                                       double host_dt, double cfl, int mode) {
+    step_ctl_block(ctl, part_dt, n_part, part_bf, n_bf, dx, dy, host_dt, cfl, mode);
 }
 
 #endif  // AMHD_API_TU
@@ -674,6 +692,75 @@ __global__ void scale_plane_kernel(double* __restrict__ x, long n, double s) {
     for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x) x[k] *= s;
 }
 
+#endif  // AMHD_API_TU
+
+
+// ------------------------------------------------------------ batched form
+// Many independent instances advanced by one persistent launch per stage.
+// A task is one (instance, strip, segment) wave of work; the persistent grid
+// (one wave per resident slot) walks the task list with a fixed stride.
+// Each instance keeps its own StepCtl (dt, ch, psi decay, t, step, done,
+// current state buffer), so the captured step replays for every instance.
+struct BatchInst {
+    void* buf[3];          // state buffers (interior layout)
+    double* part_dt;       // [n_strips * n_segs][4]
+    double* part_bf;       // [n_strips][2]
+    StepCtl* ctl;
+    long plane;
+    int pitch;
+    unsigned bytes;
+    int nx, ny, bcx_periodic, bcy_periodic, powell;
+    int n_strips, n_segs, task_base, pad;
+    double gamma, dx, dy;
+};
+
+template <class real, class P, int STAGE>
+__global__ __launch_bounds__(WAVE* WPB) void batch_stage_kernel(const BatchInst* __restrict__ insts, int n_inst,
+                                                                int n_tasks) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int first = __builtin_amdgcn_readfirstlane(xcd_logical_block() * WPB + (int)(threadIdx.x >> 6));
+    const int stride = gridDim.x * WPB;
+    for (int task = first; task < n_tasks; task += stride) {
+        // instance holding this task: last k with task_base <= task
+        int lo = 0, hi = n_inst - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (insts[mid].task_base <= task) lo = mid; else hi = mid - 1;
+        }
+        const BatchInst& d = insts[lo];
+        const int cur = d.ctl->cur;
+        StageParams<real> a;
+        a.u0 = (const real*)d.buf[cur];
+        a.us = STAGE == 1 ? (const real*)d.buf[cur] : STAGE == 2 ? (const real*)d.buf[(cur + 1) % 3]
+                                                                : (const real*)d.buf[(cur + 2) % 3];
+        a.uo = STAGE == 2 ? (real*)d.buf[(cur + 2) % 3] : (real*)d.buf[(cur + 1) % 3];
+        a.plane = a.oplane = d.plane;
+        a.pitch = a.opitch = d.pitch;
+        a.in_bytes = a.out_bytes = d.bytes;
+        a.nx = d.nx; a.ny = d.ny;
+        a.bcx_periodic = d.bcx_periodic; a.bcy_periodic = d.bcy_periodic;
+        a.n_strips = d.n_strips; a.n_segs = d.n_segs;
+        a.gamma = d.gamma; a.dx = d.dx; a.dy = d.dy;
+        a.dt = 0.0; a.ch = 0.0; a.decay = 1.0;
+        a.ctl = d.ctl;
+        a.part_dt = STAGE == 3 ? d.part_dt : nullptr;
+        a.part_bf = STAGE == 1 ? d.part_bf : nullptr;
+        const int wave = task - d.task_base;
+        if (d.powell) stage_wave<real, P, STAGE, true, true>(a, wave, lane);
+        else stage_wave<real, P, STAGE, false, true>(a, wave, lane);
+    }
+}
+
+#ifdef AMHD_API_TU
+// One workgroup per instance: close the step and compute the next dt.
+// first != 0: the partials come from the standalone CFL kernel
+// (ctl->n_part of them); otherwise from stage 3 (n_strips * n_segs).
+__global__ __launch_bounds__(256) void batch_ctl_kernel(const BatchInst* __restrict__ insts, int first) {
+    const BatchInst& d = insts[blockIdx.x];
+    const int n_part = first ? d.ctl->n_part : d.n_strips * d.n_segs;
+    step_ctl_block(d.ctl, d.part_dt, n_part, d.bcx_periodic ? nullptr : d.part_bf, d.n_strips, d.dx, d.dy, 0.0,
+                   0.0, CTL_DEVICE_LOOP);
+}
 #endif  // AMHD_API_TU
 
 }  // namespace amhd

@@ -13,6 +13,15 @@ namespace amhd {
 hipError_t launch_stage_f64(int stage, bool powell, bool glm, const StageParams<double>& a, hipStream_t s);
 hipError_t launch_stage_f32(int stage, bool powell, bool glm, const StageParams<float>& a, hipStream_t s);
 
+// Batched stage launch over a device array of BatchInst (persistent grid of
+// `resident` wavefronts).
+hipError_t launch_batch_stage_f64(int stage, const BatchInst* insts, int n_inst, int n_tasks, int resident,
+                                  hipStream_t s);
+hipError_t launch_batch_stage_f32(int stage, const BatchInst* insts, int n_inst, int n_tasks, int resident,
+                                  hipStream_t s);
+int batch_waves_per_cu_f64();
+int batch_waves_per_cu_f32();
+
 // Resident wavefronts per CU of the stage kernel (occupancy query).
 int stage_waves_per_cu_f64(bool powell);
 int stage_waves_per_cu_f32(bool powell);
@@ -66,6 +75,26 @@ hipError_t launch_stage_impl(int stage, bool powell, bool glm, const StageParams
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+template <class real, class P>
+hipError_t launch_batch_stage_impl(int stage, const BatchInst* insts, int n_inst, int n_tasks, int resident,
+                                   hipStream_t s) {
+    const dim3 grid((resident + WPB - 1) / WPB), block(WAVE * WPB);
+    switch (stage) {
+        case 1: batch_stage_kernel<real, P, 1><<<grid, block, 0, s>>>(insts, n_inst, n_tasks); break;
+        case 2: batch_stage_kernel<real, P, 2><<<grid, block, 0, s>>>(insts, n_inst, n_tasks); break;
+        case 3: batch_stage_kernel<real, P, 3><<<grid, block, 0, s>>>(insts, n_inst, n_tasks); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <class real, class P>
+int batch_waves_per_cu_impl() {
+    int nb = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_stage_kernel<real, P, 3>, WAVE * WPB, 0);
+    return e == hipSuccess && nb > 0 ? nb * WPB : WPB;
 }
 
 template <class real, class P>

@@ -94,6 +94,55 @@ def gpu_measure(name: str, steps: int, warmup: int, seed: int, rows: int | None)
     }
 
 
+def batch_measure(n: int, steps: int, warmup: int, seed: int, seq_steps: int = 20):
+    """Batched sweep: n seeded single-mode RMI instances (grids 256x128 ..
+    1024x512, Mach U[2,12], Atwood U[0.2,0.8], beta log-U[1,100]) in FP32,
+    one persistent launch per RK stage, graph-replayed; compared with the
+    same instances advanced one after another by their own device loops."""
+    import torch
+    from amhd_batch import BatchSolver
+    from amhd_ics import batched_instances, cons_from_prim_host
+    from solver import MHDSolver
+    import amhd_tune
+
+    insts = batched_instances(n=n, seed=seed, precision="fp32", max_steps=2 ** 30)
+    cases = []
+    for cfg, W in insts:
+        cfg.t_end = 1e30
+        cases.append((cfg, cons_from_prim_host(W, cfg.gamma)))
+    del insts
+    b = BatchSolver(cases)
+    rows = amhd_tune.tune_batch(b, candidates=(0, 4, 8, 16), steps=3, use_cache=False)
+    b.run(warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    took = b.run(steps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    assert int(took.min()) == steps
+    cells = b.cell_count
+    info = b.info()
+    b.close()
+    # the same instances one at a time (device loop + graph each)
+    seq_wall = 0.0
+    for cfg, U in cases:
+        s = MHDSolver(cfg)
+        s.U = U
+        s.advance(2)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        s.advance(seq_steps)
+        torch.cuda.synchronize()
+        seq_wall += time.perf_counter() - t1
+        s.ctx.close()
+    return {"workload": f"batched sweep: {n} single-mode RMI instances, grids 256x128..1024x512 (seed {seed}), "
+                        f"FP32, one persistent launch per RK stage, hipGraph replay",
+            "value": round(cells * steps / wall / 1e6, 2), "unit": "Mcell-updates/s", "dtype": "fp32",
+            "steps": steps, "ms_per_step": round(wall * 1e3 / steps, 4), "cells": cells,
+            "tasks": info["tasks"], "rows_per_task": rows or info["rows"],
+            "sequential_value": round(cells * seq_steps / seq_wall / 1e6, 2)}
+
+
 def cpu_baseline(seed: int, steps: int, threads: int):
     """The CPU oracle (C restatement of the reference math, FP64, OpenMP
     over x rows) on the 2048 x 1024 sieve: `steps` SSP-RK3 steps with the
@@ -154,6 +203,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-secondary", action="store_true")
     ap.add_argument("--copy-calib", action="store_true", help="run a 1 GiB device copy first (PMC scaling)")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-steps", type=int, default=500)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -179,6 +230,10 @@ def main():
                               "achieved_GBps": round(x["achieved_gbs"], 1),
                               "frac": round(x["achieved_gbs"] / HBM_PEAK_GBS, 4),
                               "stage_avg_us": [round(u, 2) for u in x["stage_avg_us"]]})
+
+    if not args.no_secondary and not args.no_batch:
+        log("[bench] secondary batch64")
+        secondary.append(batch_measure(64, args.batch_steps, 5, args.seed))
 
     cpu = None
     if not args.no_cpu:

@@ -136,6 +136,27 @@ int amhd_enforce_scalar_bounds(double* U, long n, void* stream);
 int amhd_compute_dt_w(const double* W, long n, double gamma, double cfl, double dx, double dy, double* dt,
                       double* ch, void* stream);
 
+/* ---- batched form (no counterpart in the reference): many independent
+ * instances of uneven size advanced together, one persistent launch per RK
+ * stage over all instances, each with its own device-side dt, glm_ch, psi
+ * decay, t and step count; the step is captured once in a hipGraph. */
+
+/* n instances; instance k has an nx[k] x ny[k] interior and params[k]. */
+int amhd_batch_create(void** batch, int n, const int* nx, const int* ny, int dtype, const amhd_params* params);
+int amhd_batch_destroy(void* batch);
+/* Rows per task (0 = sized so that the task count matches the resident
+ * wavefront slots). */
+int amhd_batch_set_rows(void* batch, int rows);
+int amhd_batch_info(void* batch, int* n_tasks, int* resident_waves, int* rows);
+/* Ghosted float64 (10, nx[k]+4, ny[k]+4) state of instance k, as
+ * amhd_upload_state / amhd_download_state. */
+int amhd_batch_upload(void* batch, int k, const double* U, int on_device);
+int amhd_batch_download(void* batch, int k, double* U, int on_device);
+/* Up to n_steps steps of every instance (each stops at its own t_end[k] /
+ * max_steps[k], counted in step_inout[k]); t_inout, step_inout: [n]. */
+int amhd_batch_run(void* batch, int n_steps, const double* cfl, const double* t_end, const int* max_steps,
+                   int use_graph, double* t_inout, int* step_inout);
+
 #ifdef __cplusplus
 }
 #endif
