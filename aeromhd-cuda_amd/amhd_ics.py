@@ -18,6 +18,7 @@ three are extensions with the parameters stated in their docstrings.
 """
 from __future__ import annotations
 
+import functools
 import math
 
 import numpy as np
@@ -194,12 +195,19 @@ def magnetic_sieve(nx: int = 2048, ny: int = 1024, precision: str = "fp32", seed
                  diag_interval=10 ** 9, snapshot_times=[], precision=precision)
     x, y = cell_centres(cfg)
     by_pre = b0 * (1.0 + 0.5 * np.sin(2 * np.pi * y / lam_b))
-    post = np.array([mhd_rankine_hugoniot(cfg.gamma, mach, 1.0, 1.0, float(b))[:4] for b in by_pre])
+    post = _row_jumps(cfg.gamma, mach, tuple(float(b) for b in by_pre))
     x_if, *_ = sieve_interface(y, ly, cfg.interface_x, 0.02 * lam, seed)
     X = np.broadcast_to(x[:, None], (x.size, y.size))
     rows = (post[:, 0][None, :], post[:, 1][None, :], post[:, 2][None, :], post[:, 3][None, :])
     W = _rmi_fill(cfg, _prim(cfg), X, x_if[None, :], rows, by_pre[None, :], cfg.density_ratio)
     return cfg, W
+
+
+@functools.lru_cache(maxsize=8)
+def _row_jumps(gamma: float, mach: float, by_rows: tuple) -> np.ndarray:
+    """Post-shock (rho2, p2, vx2, By2) for each row's pre-shock By (the
+    profile does not depend on the seed, so the solves are cached)."""
+    return np.array([mhd_rankine_hugoniot(gamma, mach, 1.0, 1.0, b)[:4] for b in by_rows])
 
 
 def batched_instances(n: int = 64, seed: int = 0, precision: str = "fp32", max_steps: int = 500,

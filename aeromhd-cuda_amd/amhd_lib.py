@@ -50,6 +50,10 @@ SIGNATURES = {
     "amhd_reset_boundary_energy": (_i, [_vp, _d]),
     "amhd_compute_rhs": (_i, [_vp, _vp, _vp, _d]),
     "amhd_global_sums": (_i, [_vp, _dp]),
+    "amhd_set_source": (_i, [_vp, _vp]),
+    "amhd_rewind": (_i, [_vp]),
+    "amhd_state_check": (_i, [_vp, _dp]),
+    "amhd_state_ptr": (_i, [_vp, C.POINTER(_vp)]),
     "amhd_cons_to_prim": (_i, [_vp, _vp, _l, _d, _vp]),
     "amhd_prim_to_cons": (_i, [_vp, _vp, _l, _d, _vp]),
     "amhd_muscl": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp]),
@@ -64,6 +68,11 @@ SIGNATURES = {
     "amhd_batch_upload": (_i, [_vp, _i, _vp, _i]),
     "amhd_batch_download": (_i, [_vp, _i, _vp, _i]),
     "amhd_batch_run": (_i, [_vp, _i, _dp, _dp, _ip, _i, _dp, _ip]),
+    "amhd_conv3x3_kpad": (_i, [_i]),
+    "amhd_conv3x3_fwd": (_i, [_vp, _i, _i, _i, _i, _vp, _vp, _i, _i, _vp, _i, _vp]),
+    "amhd_conv3x3_bwd_data": (_i, [_vp, _i, _i, _i, _i, _vp, _i, _vp, _vp, _vp]),
+    "amhd_conv3x3_bwd_weight": (_i, [_vp, _vp, _i, _i, _i, _i, _i, _vp, _vp, _vp, _i, _vp]),
+    "amhd_mfma_probe": (_i, [_vp, _vp, _vp, _vp]),
 }
 
 _lib = None

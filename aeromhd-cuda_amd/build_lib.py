@@ -7,7 +7,8 @@ Three translation units, compiled by hipcc in-tree:
 * ``csrc/amhd_f32.hip``  FP32 fused update, ``-ffp-contract=on`` (multiply-adds fused per
   source expression, so a body inlined into different kernels rounds the same);
 * ``csrc/amhd_api.hip``  the C ABI of ``include/aeromhd.h`` and the FP64
-  standalone operators, ``-ffp-contract=off``.
+  standalone operators, ``-ffp-contract=off``;
+* ``csrc/amhd_cnn.hip``  BF16-MFMA 3x3 convolutions of the learned closure.
 
 Every TU is built with packed-FP32 instructions disabled
 (``-target-feature -packed-fp32-ops``).
@@ -36,6 +37,7 @@ UNITS = [
     ("amhd_f64.hip", ["-ffp-contract=off"]),
     ("amhd_f32.hip", ["-ffp-contract=on"]),
     ("amhd_api.hip", ["-ffp-contract=off"]),
+    ("amhd_cnn.hip", ["-ffp-contract=on"]),
 ]
 
 

@@ -32,6 +32,15 @@ int fail(int code, const char* fmt, ...) {
     return code;
 }
 
+}  // namespace
+
+int amhd_set_error(int code, const char* text) {
+    g_err = text;
+    return code;
+}
+
+namespace {
+
 #define HIP_TRY(expr)                                                                          \
     do {                                                                                       \
         hipError_t e_ = (expr);                                                                \
@@ -84,6 +93,7 @@ struct Ctx {
     StepCtl* ctl = nullptr;
     hipStream_t stream = nullptr;
     hipGraphExec_t graph[3] = {nullptr, nullptr, nullptr};
+    const void* src = nullptr;   // optional per-cell source (interior layout, context precision)
 };
 
 size_t esize(int dtype) { return dtype == AMHD_F32 ? sizeof(float) : sizeof(double); }
@@ -114,6 +124,7 @@ StageParams<real> stage_args(const Ctx* c, int stage, const real* u0, const real
     a.ctl = nullptr;
     a.part_dt = stage == 3 ? c->part_dt : nullptr;
     a.part_bf = stage == 1 ? c->part_bf : nullptr;
+    a.src = (const real*)c->src;
     return a;
 }
 
@@ -531,6 +542,54 @@ int amhd_compute_rhs(void* ctx, double* U, double* R, double ch) {
     a.dt = 0.0; a.ch = ch; a.decay = 1.0;
     HIP_TRY(launch_stage_f64(0, c->p.powell != 0, ch > 0.0, a, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int amhd_set_source(void* ctx, const void* src) {
+    if (!ctx) return fail(-1, "amhd_set_source: null context");
+    Ctx* c = (Ctx*)ctx;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (c->src != src) drop_graphs(c);
+    c->src = src;
+    return 0;
+}
+
+int amhd_rewind(void* ctx) {
+    if (!ctx) return fail(-1, "amhd_rewind: null context");
+    Ctx* c = (Ctx*)ctx;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->cur = (c->cur + 2) % 3;
+    c->part_valid = false;
+    return 0;
+}
+
+int amhd_state_check(void* ctx, double out[4]) {
+    if (!ctx || !out) return fail(-1, "amhd_state_check: null argument");
+    Ctx* c = (Ctx*)ctx;
+    const int g = grid_for((long)c->nx * c->ny, 256, GSUM_BLOCKS);
+    if (c->dtype == AMHD_F64)
+        state_check_kernel<double><<<g, 256, 0, c->stream>>>((const double*)c->buf[c->cur], c->plane, c->pitch, c->nx,
+                                                             c->ny, c->p.gamma, c->p.dx, c->p.dy, c->gsum);
+    else
+        state_check_kernel<float><<<g, 256, 0, c->stream>>>((const float*)c->buf[c->cur], c->plane, c->pitch, c->nx,
+                                                            c->ny, c->p.gamma, c->p.dx, c->p.dy, c->gsum);
+    HIP_TRY(hipGetLastError());
+    double h[4 * GSUM_BLOCKS];
+    HIP_TRY(hipMemcpyAsync(h, c->gsum, sizeof(double) * 4 * g, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    out[0] = h[0]; out[1] = h[1]; out[2] = h[2]; out[3] = h[3];
+    for (int k = 1; k < g; ++k) {
+        out[0] = std::min(out[0], h[4 * k]); out[1] = std::min(out[1], h[4 * k + 1]);
+        out[2] = std::max(out[2], h[4 * k + 2]); out[3] += h[4 * k + 3];
+    }
+    return 0;
+}
+
+int amhd_state_ptr(void* ctx, void** state) {
+    if (!ctx || !state) return fail(-1, "amhd_state_ptr: null argument");
+    Ctx* c = (Ctx*)ctx;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    *state = c->buf[c->cur];
     return 0;
 }
 

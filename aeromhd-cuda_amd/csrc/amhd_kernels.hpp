@@ -54,6 +54,7 @@ template <class real> struct StageParams {
     const StepCtl* ctl;     // device-resident dt/ch/decay
     double* part_dt;        // STAGE 3: [n_waves][4]
     double* part_bf;        // STAGE 1: [n_strips][2] (non-periodic x only)
+    const real* src;        // optional per-cell source added to L(U) (same layout as us), or null
 };
 
 // ---------------------------------------------------------- buffer access
@@ -299,6 +300,7 @@ __device__ __forceinline__ void stage_wave(const StageParams<real>& a, const int
                 if (v == EN) s = -divb * (w0[iVX] * w0[iBX] + w0[iVY] * w0[iBY] + w0[iVZ] * w0[iBZ]);
                 r = r + s;
             }
+            if (a.src) r = r + a.src[(long)v * a.plane + (long)i * a.pitch + sj];
             if constexpr (STAGE == 0) unew[v] = r;
             else if constexpr (STAGE == 1) unew[v] = ucur[v] + dtr * r;
             else if constexpr (STAGE == 2) unew[v] = real(0.75) * ubase[v] + real(0.25) * (ucur[v] + dtr * r);
@@ -656,6 +658,54 @@ __global__ __launch_bounds__(256) void hlld_kernel(const real* __restrict__ WL, 
 }
 
 #ifdef AMHD_API_TU
+// min rho, min p, max |divB| (interior stencil of solver.py:419-428) and a
+// non-finite count over an interior-layout state; one partial per workgroup.
+template <class real>
+__global__ __launch_bounds__(256) void state_check_kernel(const real* __restrict__ u, long plane, int pitch, int nx,
+                                                          int ny, double gamma, double dx, double dy, double* part) {
+    double mr = 1e300, mp = 1e300, md = 0.0, nf = 0.0;
+    const long n = (long)nx * ny;
+    for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(k / ny), j = (int)(k - (long)i * ny);
+        const long o = (long)i * pitch + j;
+        double c[NV];
+        bool fin = true;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) { c[v] = double(u[v * plane + o]); fin = fin && isfinite(c[v]); }
+        if (!fin) nf += 1.0;
+        const double rho = c[RHO];
+        const double ke = 0.5 * (c[MX] * c[MX] + c[MY] * c[MY] + c[MZ] * c[MZ]) / rho;
+        const double me = 0.5 * (c[BX] * c[BX] + c[BY] * c[BY] + c[BZ] * c[BZ]);
+        const double p = (gamma - 1.0) * (c[EN] - ke - me);
+        mr = fmin(mr, rho);
+        mp = fmin(mp, p);
+        if (i >= 1 && i < nx - 1 && j >= 1 && j < ny - 1) {
+            const real* bx = u + BX * plane;
+            const real* by = u + BY * plane;
+            const double d = (0.5 * (double(bx[o]) + double(bx[o + pitch])) - 0.5 * (double(bx[o - pitch]) + double(bx[o]))) / dx +
+                             (0.5 * (double(by[o]) + double(by[o + 1])) - 0.5 * (double(by[o - 1]) + double(by[o]))) / dy;
+            md = fmax(md, fabs(d));
+        }
+    }
+    __shared__ double sh[4][4];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int off = 32; off > 0; off >>= 1) {
+        mr = fmin(mr, __shfl_xor(mr, off, 64));
+        mp = fmin(mp, __shfl_xor(mp, off, 64));
+        md = fmax(md, __shfl_xor(md, off, 64));
+        nf += __shfl_xor(nf, off, 64);
+    }
+    if (lane == 0) { sh[0][w] = mr; sh[1][w] = mp; sh[2][w] = md; sh[3][w] = nf; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 1; q < (int)(blockDim.x >> 6); ++q) {
+            sh[0][0] = fmin(sh[0][0], sh[0][q]); sh[1][0] = fmin(sh[1][0], sh[1][q]);
+            sh[2][0] = fmax(sh[2][0], sh[2][q]); sh[3][0] += sh[3][q];
+        }
+        for (int q = 0; q < 4; ++q) part[blockIdx.x * 4 + q] = sh[q][0];
+    }
+}
+
 // solver.py:158-197 in place on a ghosted f64 array.  Every ghost value is
 // an image of an interior cell, so one pass over the ghost cells suffices.
 __global__ void apply_bc_kernel(double* __restrict__ U, int nx, int ny, int bcx_periodic, int bcy_periodic) {
@@ -743,6 +793,7 @@ __global__ __launch_bounds__(WAVE* WPB) void batch_stage_kernel(const BatchInst*
         a.gamma = d.gamma; a.dx = d.dx; a.dy = d.dy;
         a.dt = 0.0; a.ch = 0.0; a.decay = 1.0;
         a.ctl = d.ctl;
+        a.src = nullptr;
         a.part_dt = STAGE == 3 ? d.part_dt : nullptr;
         a.part_bf = STAGE == 1 ? d.part_bf : nullptr;
         const int wave = task - d.task_base;

@@ -106,6 +106,20 @@ int amhd_compute_rhs(void* ctx, double* U_dev, double* R_dev, double ch);
  * out[3] = sqrt(mean(divB^2)) with divB as solver.py:419-428. */
 int amhd_global_sums(void* ctx, double out[4]);
 
+/* Learned-closure hooks (no counterpart in the reference).
+ * amhd_set_source: a per-cell source S (10 planes of nx*ny cells, the
+ * context's precision, device memory, or NULL) added to L(U) in every stage
+ * of subsequent steps until cleared -- over one SSP-RK3 step it adds dt*S.
+ * amhd_rewind: after a host-driven step, make the previous state current
+ * again (the step is discarded).
+ * amhd_state_check: out = {min rho, min p, max|divB|, non-finite cells}.
+ * amhd_state_ptr: device address of the current interior-layout state
+ * (valid until the next step). */
+int amhd_set_source(void* ctx, const void* src);
+int amhd_rewind(void* ctx);
+int amhd_state_check(void* ctx, double out[4]);
+int amhd_state_ptr(void* ctx, void** state);
+
 /* ---- standalone operators of physics.py / solver.py (float64, device
  * pointers, asynchronous on `stream` unless a host result is returned) */
 
@@ -156,6 +170,29 @@ int amhd_batch_download(void* batch, int k, double* U, int on_device);
  * max_steps[k], counted in step_inout[k]); t_inout, step_inout: [n]. */
 int amhd_batch_run(void* batch, int n_steps, const double* cfl, const double* t_end, const int* max_steps,
                    int use_graph, double* t_inout, int* step_inout);
+
+/* ---- learned flux closure (no counterpart in the reference): 3x3
+ * convolutions on BF16 matrix cores (v_mfma_f32_16x16x32_bf16) with FP32
+ * accumulation.  NHWC tensors over the coarse grid, (n, nx, ny, C) with
+ * C in {16, 32}; zero padding in x, periodic in y.  Weights W[co][k],
+ * k = tap*Cin + ci, tap = (di+1)*3 + (dj+1), padded to kpad = roundup(9*Cin, 32);
+ * transposed weights WT[ci][tap*Cout + co] padded to roundup(9*Cout, 32). */
+int amhd_conv3x3_kpad(int cin);
+/* y = act(conv(x, W) + bias): BF16 x, W; y BF16 (relu = 1) or FP32 (y_f32 = 1). */
+int amhd_conv3x3_fwd(const void* x, int n, int nx, int ny, int cin, const void* w, const float* bias, int cout,
+                     int relu, void* y, int y_f32, void* stream);
+/* dx = mask * convT(dy, W): dy FP32 (n,nx,ny,cout); mask BF16 (n,nx,ny,cin)
+ * or NULL (dx where mask > 0); dx FP32. */
+int amhd_conv3x3_bwd_data(const float* dy, int n, int nx, int ny, int cout, const void* wt, int cin, const void* mask,
+                          float* dx, void* stream);
+/* dW[co][k] = sum_p dy[p][co] x[p + off(tap)][ci], dbias = sum_p dy; the
+ * pixel sum is split over n_split workgroup columns with FP32 partials in
+ * workspace (n_split * cout * kpad floats), summed in a fixed order. */
+int amhd_conv3x3_bwd_weight(const void* x, const float* dy, int n, int nx, int ny, int cin, int cout, float* dw,
+                            float* dbias, float* workspace, int n_split, void* stream);
+/* D (16x16) = A (16x32) * B (32x16), row-major FP32, operands rounded to BF16:
+ * one v_mfma_f32_16x16x32_bf16 (checks the lane layout). */
+int amhd_mfma_probe(const float* A, const float* B, float* D, void* stream);
 
 #ifdef __cplusplus
 }
