@@ -4,10 +4,9 @@ in the reference).
 A 4-layer 3x3 CNN (32 channels; BF16 weights, FP32 accumulation, run by
 the BF16-MFMA kernels of libaeromhd_hip.so) maps the coarse conserved
 state -- 9 fields: rho, m, B, E, rhoC -- to corrections of the x- and
-y-face fluxes of those 9 fields.  Their discrete divergence enters the
-coarse SSP-RK3 step as a constant source S = -div(dF) in every stage
-(amhd_set_source), which adds dt*S per step and, being a flux difference,
-conserves every corrected quantity exactly.  Face corrections on the two
+y-face fluxes of those 9 fields.  Their discrete divergence S = -div(dF)
+is added to the coarse step as dt*S and, being a flux difference, conserves
+every corrected quantity exactly.  Face corrections on the two
 x boundaries are held at zero; y is periodic.
 
 Training data come from this solver: fine 2048x1024 magnetic-sieve runs
@@ -15,12 +14,25 @@ Training data come from this solver: fine 2048x1024 magnetic-sieve runs
 variables.  For a pair of restricted states (U_a at t, U_b at t + dt,
 dt = 8 fine steps) the target is the source the plain coarse step misses:
 r = (U_b - RK3_coarse(U_a, dt)) / dt.  The loss is the normalised mean
-squared error between S(CNN(U_a)) and r.
+squared error between S(CNN(U_a)) and r.  The rhoC face flux is not
+predicted freely: it is C_donor * F_rho (the concentration upwind of the
+mass-flux correction, taken from the state the correction is added to)
+plus the network's mixing flux G, so that a mass correction carries its
+concentration and C cannot leak into pure fluid.  A model may apply only
+some of the corrections (stats["fields"]; the shipped one applies the
+mixing flux G alone): it is trained and run with the others held at zero.
+After the first fit, rounds of data aggregation add pairs taken along the
+corrected solver's own trajectories (rollout_pairs).
 
-At run time every corrected step is checked (density and pressure
-positive, no non-finite values, max|divB| not above 1.5x its pre-step
-value plus a floor); a failing step is discarded (amhd_rewind) and redone
-by the plain coarse solver.
+At run time a corrected step is the plain coarse SSP-RK3 step followed by
+U += dt*S(U_pre) (the form of the target), applied in place by
+amhd_closure_source through a positivity limiter: per cell the largest
+theta in {1, 1/2, .., 1/32, 0} for which the post-step state plus
+theta*dt*S keeps half of its density and pressure, and each face takes the
+smaller theta of its two cells, so the limited correction is still a flux
+difference.  Every corrected step is then checked (density and pressure positive, no non-finite values,
+max|divB| not above 1.5x its pre-step value plus a floor); a failing step
+is discarded (amhd_rewind) and redone by the plain coarse solver.
 
 The fine-grid solver judges the result: integral mixing width and L1
 density error of the coarse run against the restricted fine run.
@@ -90,23 +102,35 @@ class FluxCNN:
     def parameters(self):
         return self.W + self.b
 
-    def forward(self, x, keep: bool = False):
+    def forward(self, x, keep: bool = False, stream=None, bufs=None):
         """x: (n, nx, ny, 16) BF16.  Returns the (n, nx, ny, 32) FP32 output
-        (channels 0..17 used) and, with keep, the layer inputs."""
+        (channels 0..17 used) and, with keep, the layer inputs.  `bufs`
+        (from alloc_outputs) and `stream` let a caller run the layers on
+        its own stream into memory it owns."""
         torch = _L.require_gpu()
         n, nx, ny, _ = x.shape
         acts = [x]
-        stream = _L.stream_handle(torch)
+        if stream is None:
+            stream = _L.stream_handle(torch)
+        if bufs is None:
+            bufs = self.alloc_outputs(n, nx, ny, x.device)
         h = x
         for li, (cin, cout, relu) in enumerate(LAYERS):
             last = li == len(LAYERS) - 1
-            y = torch.empty(n, nx, ny, cout, device=x.device, dtype=torch.float32 if last else torch.bfloat16)
+            y = bufs[li]
             _L.check(self.lib.amhd_conv3x3_fwd(_p(h), n, nx, ny, cin, _p(self.Wb[li]), _p(self.b[li]), cout,
                                                1 if relu else 0, _p(y), 1 if last else 0, stream), "amhd_conv3x3_fwd")
             h = y
             if keep and not last:
                 acts.append(y)
         return (h, acts) if keep else h
+
+    @staticmethod
+    def alloc_outputs(n, nx, ny, device="cuda"):
+        torch = _L.require_gpu()
+        return [torch.empty(n, nx, ny, cout, device=device,
+                            dtype=torch.float32 if li == len(LAYERS) - 1 else torch.bfloat16)
+                for li, (_, cout, _) in enumerate(LAYERS)]
 
     def backward(self, acts, dout, n_split: int = 64):
         """Gradients of every W and b for d(loss)/d(output) = dout (FP32,
@@ -158,26 +182,59 @@ class FluxCNN:
 
 
 # ------------------------------------------------------- features / source
-def features(Uc, stats):
-    """(n, 10, nx, ny) coarse conserved (device) -> normalised NHWC BF16
-    (n, nx, ny, 16) with the 9 FEAT fields in channels 0..8."""
+PAIR_FIELDS = list(range(N_IN))   # planes of the network fields in a (n, 9, nx, ny) pair tensor
+
+
+def _int9(v):
+    return (C.c_int * N_IN)(*[int(a) for a in v])
+
+
+def _f9(v):
+    return (C.c_float * N_IN)(*[float(a) for a in v])
+
+
+def features(states, dtype: int, fields, nx: int, ny: int, stats, out, stream=None):
+    """Normalised NHWC BF16 network input (n, nx, ny, 16) of n device arrays
+    (`states`: addresses of 10- or 9-plane arrays of nx*ny cells, `fields`:
+    the plane of each network field), by amhd_closure_features."""
     torch = _L.require_gpu()
-    mean = torch.as_tensor(stats["in_mean"], device=Uc.device, dtype=torch.float32).view(1, -1, 1, 1)
-    std = torch.as_tensor(stats["in_std"], device=Uc.device, dtype=torch.float32).view(1, -1, 1, 1)
-    x = (Uc[:, FEAT].float() - mean) / std
-    out = torch.zeros(Uc.shape[0], Uc.shape[2], Uc.shape[3], 16, device=Uc.device, dtype=torch.bfloat16)
-    out[..., :N_IN] = x.permute(0, 2, 3, 1).to(torch.bfloat16)
+    n = len(states)
+    ptrs = (C.c_void_p * n)(*states)
+    _L.check(_L.load().amhd_closure_features(ptrs, dtype, _int9(fields), n, nx, ny, _f9(stats["in_mean"]),
+                                             _f9(stats["in_std"]), _p(out),
+                                             stream if stream is not None else _L.stream_handle(torch)),
+             "amhd_closure_features")
     return out
 
 
-def source_from_flux(dF, dx: float, dy: float):
-    """S = -div(dF) for face corrections dF (n, nx, ny, 18): channels 0..8
-    at the x face i-1/2 of cell i (zero on both x boundaries), 9..17 at the
-    y face j-1/2 (periodic).  Returns (n, nx, ny, 9)."""
+def pair_features(X, stats):
+    """Network input of training states X (n, 9, nx, ny) FP32."""
+    torch = _L.require_gpu()
+    X = X.contiguous()
+    n, _, nx, ny = X.shape
+    out = torch.empty(n, nx, ny, 16, device=X.device, dtype=torch.bfloat16)
+    step = N_IN * nx * ny * X.element_size()
+    return features([X.data_ptr() + m * step for m in range(n)], 1, PAIR_FIELDS, nx, ny, stats, out)
+
+
+RC = N_IN - 1        # position of rhoC among the network fields
+
+
+def source_from_flux(dF, dx: float, dy: float, C):
+    """S = -div of the face corrections dF (n, nx, ny, 18) in physical
+    units: channels 0..8 at the x face i-1/2 of cell i (zero on both x
+    boundaries), 9..17 at the y face j-1/2 (periodic).  The rhoC channels
+    carry the mixing flux G; the rhoC face flux is C_donor * F_rho + G with
+    the donor upwind of F_rho and C (n, nx, ny) the concentration of the
+    state the correction is added to.  Returns (n, nx, ny, 9)."""
     torch = _L.require_gpu()
     fx = dF[..., :N_IN]
-    fx = torch.cat([torch.zeros_like(fx[:, :1]), fx[:, 1:], torch.zeros_like(fx[:, :1])], dim=1)
     fy = dF[..., N_IN:N_OUT]
+    Cx = torch.where(fx[..., 0] > 0, torch.roll(C, 1, dims=1), C)
+    Cy = torch.where(fy[..., 0] > 0, torch.roll(C, 1, dims=2), C)
+    fx = torch.cat([fx[..., :RC], (fx[..., RC] + Cx * fx[..., 0]).unsqueeze(-1)], dim=-1)
+    fy = torch.cat([fy[..., :RC], (fy[..., RC] + Cy * fy[..., 0]).unsqueeze(-1)], dim=-1)
+    fx = torch.cat([torch.zeros_like(fx[:, :1]), fx[:, 1:], torch.zeros_like(fx[:, :1])], dim=1)
     sx = -(fx[:, 1:] - fx[:, :-1]) / dx
     sy = -(torch.roll(fy, -1, dims=2) - fy) / dy
     return sx + sy
@@ -229,12 +286,13 @@ def fine_run(seed: int, nx: int = 2048, ny: int = 1024, steps: int = 2000, on_sa
 
 def generate_pairs(seeds, pairs_per_run: int = 16, span: int = 8, steps: int = 2000, nx: int = 2048,
                    ny: int = 1024, log=None):
-    """Training pairs (U_a[FEAT], r[FEAT]) on the coarse grid from fine runs."""
+    """Training pairs on the coarse grid from fine runs: (X, R, Cp) =
+    (U_a[FEAT], r[FEAT], C of RK3_coarse(U_a)), stacked on the device."""
     torch = _L.require_gpu()
     from solver import MHDSolver
 
     starts = [int(v) for v in np.linspace(0, steps - span, pairs_per_run)]
-    X, R = [], []
+    X, R, CP = [], [], []
     coarse = None
     for si, seed in enumerate(seeds):
         t0 = time.time()
@@ -257,12 +315,72 @@ def generate_pairs(seeds, pairs_per_run: int = 16, span: int = 8, steps: int = 2
             Uc = coarse.ctx.download()[:, 2:-2, 2:-2]
             X.append(Ua[FEAT].float())
             R.append(((Ub - Uc) / dt)[FEAT].float())
+            CP.append((Uc[9] / Uc[0]).float())
             state["n"] = state.get("n", 0) + 1
 
         fine_run(seed, nx, ny, steps, sample, starts)
         if log:
             log(f"[closure] data seed {seed}: {state.get('n', 0)} pairs, {time.time() - t0:.1f} s")
-    return torch.stack(X), torch.stack(R)
+    return torch.stack(X), torch.stack(R), torch.stack(CP)
+
+
+def rollout_pairs(model, seeds, starts: int = 10, horizon: int = 25, span: int = 8, steps: int = 2000,
+                  nx: int = 2048, ny: int = 1024, keep: float = 0.5, pullback: float = 0.1, log=None):
+    """Pairs along the closure's own coarse trajectories (data aggregation):
+    the fine run is restricted every `span` fine steps (F_k at t_k); from
+    `starts` points F_k0 the corrected coarse solver steps with
+    dt_k = t_{k+1} - t_k for `horizon` steps, and step k gives the pair
+    (V_k, r_k, C of RK3(V_k)) with the tendency-matching target
+    r_k = [(F_{k+1} - F_k) - (RK3(V_k) - V_k) + pullback * (F_k - V_k)] / dt_k:
+    the fine solution's change minus the coarse step's own change at the
+    visited state, plus a fraction of the drift (all of it, pullback = 1,
+    asks one step to undo the accumulated error, which V_k alone cannot
+    tell; at V_k = F_k every pullback gives the first-fit target)."""
+    torch = _L.require_gpu()
+    X, R, CP = [], [], []
+    for seed in seeds:
+        t0 = time.time()
+        F, T = [], []
+
+        def grab(s):
+            F.append(restrict(s.ctx.download()).float())
+            T.append(s.t)
+
+        fs = fine_run(seed, nx, ny, steps, grab, range(0, steps + 1, span))
+        cs = ClosureSolver(coarse_config(fs.cfg), model, keep=keep)
+        n0 = len(X)
+        for k0 in np.linspace(0, len(F) - horizon - 1, starts).astype(int):
+            cs.s.U = ghosted(F[k0].double())
+            cs.s.t = T[k0]
+            cs.begin()
+            for k in range(k0, k0 + horizon):
+                dt = T[k + 1] - T[k]
+                if dt > cs.s.compute_dt():       # the coarse CFL step is shorter: stop here
+                    break
+                V = cs.s.ctx.download()[:, 2:-2, 2:-2]
+                post = {}
+
+                def grab_plain():
+                    post["U"] = cs.s.ctx.download()[:, 2:-2, 2:-2]
+
+                cs.step(dt, on_plain=grab_plain)
+                Uc = post["U"]
+                X.append(V[FEAT].float())
+                Fk, Fk1 = F[k].double(), F[k + 1].double()
+                R.append(((Fk1 - Fk - (Uc - V) + pullback * (Fk - V)) / dt)[FEAT].float())
+                CP.append((Uc[9] / Uc[0]).float())
+                cs.s.t += dt
+        del F
+        if log:
+            log(f"[closure] rollout seed {seed}: {len(X) - n0} pairs, {cs.fallbacks} fallbacks, "
+                f"{time.time() - t0:.1f} s")
+    return torch.stack(X), torch.stack(R), torch.stack(CP)
+
+
+def join(a, b):
+    """Concatenate two pair sets (X, R, Cp)."""
+    torch = _L.require_gpu()
+    return tuple(torch.cat([x, y]) for x, y in zip(a, b))
 
 
 def fit_stats(X, R, dx: float) -> dict:
@@ -273,35 +391,44 @@ def fit_stats(X, R, dx: float) -> dict:
 
 
 # ---------------------------------------------------------------- training
-def loss_and_grad(model, X, R, stats, dx, dy):
-    """Normalised MSE of S(CNN(X)) against R and d(loss)/d(output)."""
+def applied_fields(stats) -> list:
+    """The network fields whose corrections a model applies (all by default)."""
+    f = stats.get("fields") if stats is not None else None
+    return list(range(N_IN)) if f is None else [int(v) for v in np.asarray(f).ravel()]
+
+
+def _scales(stats, device):
+    """Per-channel output scale with the fields a model does not apply zeroed,
+    and the normalising r_std."""
     torch = _L.require_gpu()
-    Ug = torch.zeros(X.shape[0], 10, X.shape[2], X.shape[3], device=X.device)
-    Ug[:, FEAT] = X
-    x = features(Ug, stats)
-    out, acts = model.forward(x, keep=True)
+    mask = np.zeros(N_IN, dtype=np.float32)
+    mask[applied_fields(stats)] = 1.0
+    fs = np.asarray(stats["f_scale"], dtype=np.float32) * mask
+    fscale = torch.as_tensor(np.concatenate([fs, fs]), device=device, dtype=torch.float32)
+    rstd = torch.as_tensor(stats["r_std"], device=device, dtype=torch.float32)
+    return fscale, rstd
+
+
+def loss_and_grad(model, X, R, Cp, stats, dx, dy):
+    """Normalised MSE of S(CNN(X)) against R over the applied fields and
+    d(loss)/d(output)."""
+    torch = _L.require_gpu()
+    out, acts = model.forward(pair_features(X, stats), keep=True)
     o = out.detach().requires_grad_(True)
-    fscale = torch.as_tensor(np.concatenate([stats["f_scale"], stats["f_scale"]]), device=X.device,
-                             dtype=torch.float32)
-    S = source_from_flux(o[..., :N_OUT] * fscale, dx, dy)
-    rstd = torch.as_tensor(stats["r_std"], device=X.device, dtype=torch.float32)
-    err = (S - R.permute(0, 2, 3, 1)) / rstd
+    fscale, rstd = _scales(stats, X.device)
+    S = source_from_flux(o[..., :N_OUT] * fscale, dx, dy, Cp)
+    err = ((S - R.permute(0, 2, 3, 1)) / rstd)[..., applied_fields(stats)]
     loss = (err * err).mean()
     loss.backward()
     return float(loss.detach()), o.grad, acts
 
 
-def loss_only(model, X, R, stats, dx, dy) -> float:
+def loss_only(model, X, R, Cp, stats, dx, dy) -> float:
     """The loss of loss_and_grad without the gradient."""
-    torch = _L.require_gpu()
-    Ug = torch.zeros(X.shape[0], 10, X.shape[2], X.shape[3], device=X.device)
-    Ug[:, FEAT] = X
-    out = model.forward(features(Ug, stats))
-    fscale = torch.as_tensor(np.concatenate([stats["f_scale"], stats["f_scale"]]), device=X.device,
-                             dtype=torch.float32)
-    S = source_from_flux(out[..., :N_OUT] * fscale, dx, dy)
-    rstd = torch.as_tensor(stats["r_std"], device=X.device, dtype=torch.float32)
-    err = (S - R.permute(0, 2, 3, 1)) / rstd
+    out = model.forward(pair_features(X, stats))
+    fscale, rstd = _scales(stats, X.device)
+    S = source_from_flux(out[..., :N_OUT] * fscale, dx, dy, Cp)
+    err = ((S - R.permute(0, 2, 3, 1)) / rstd)[..., applied_fields(stats)]
     return float((err * err).mean())
 
 
@@ -323,17 +450,25 @@ class Adam:
             p.sub_(self.lr * mh / (vh.sqrt() + self.eps))
 
 
-def train(X, R, dx, dy, iters: int = 2000, batch: int = 16, lr: float = 1e-3, seed: int = 0, log=None,
-          holdout=None):
+def train(data, dx, dy, iters: int = 2000, batch: int = 16, lr: float = 1e-3, seed: int = 0, log=None,
+          holdout=None, model=None, fields=None):
+    """Adam on the pairs data = (X, R, Cp) (generate_pairs); a given model
+    is trained further with its own normalisation.  fields: the network
+    fields whose corrections the model applies (None: all)."""
     torch = _L.require_gpu()
-    model = FluxCNN(seed)
-    model.stats = fit_stats(X, R, dx)
+    X, R, Cp = data
+    if model is None:
+        model = FluxCNN(seed)
+        model.stats = fit_stats(X, R, dx)
+        if fields is not None:
+            model.stats["fields"] = np.asarray(sorted(fields), dtype=np.int32)
     opt = Adam(model.parameters(), lr=lr)
     g = torch.Generator(device="cpu").manual_seed(seed)
     hist = []
     for it in range(iters):
+        opt.lr = lr * 0.5 * (1.0 + math.cos(math.pi * it / iters))     # cosine decay
         idx = torch.randint(0, X.shape[0], (batch,), generator=g)
-        loss, dout, acts = loss_and_grad(model, X[idx], R[idx], model.stats, dx, dy)
+        loss, dout, acts = loss_and_grad(model, X[idx], R[idx], Cp[idx], model.stats, dx, dy)
         gW, gb = model.backward(acts, dout)
         opt.step(gW + gb)
         for W, m in zip(model.W, model.wmask):
@@ -342,17 +477,18 @@ def train(X, R, dx, dy, iters: int = 2000, batch: int = 16, lr: float = 1e-3, se
         if it % 100 == 0 or it == iters - 1:
             rec = {"iter": it, "loss": loss}
             if holdout is not None:
-                rec["holdout"] = eval_loss(model, holdout[0], holdout[1], dx, dy)
+                rec["holdout"] = eval_loss(model, holdout, dx, dy)
             hist.append(rec)
             if log:
                 log(f"[closure] iter {it} {rec}")
     return model, hist
 
 
-def eval_loss(model, X, R, dx, dy, batch: int = 16) -> float:
+def eval_loss(model, data, dx, dy, batch: int = 16) -> float:
+    X, R, Cp = data
     tot, n = 0.0, 0
     for k in range(0, X.shape[0], batch):
-        l = loss_only(model, X[k:k + batch], R[k:k + batch], model.stats, dx, dy)
+        l = loss_only(model, X[k:k + batch], R[k:k + batch], Cp[k:k + batch], model.stats, dx, dy)
         tot += l * min(batch, X.shape[0] - k)
         n += min(batch, X.shape[0] - k)
     return tot / max(n, 1)
@@ -362,77 +498,152 @@ def baseline_loss(R, stats) -> float:
     """Loss of the zero correction (the plain coarse solver)."""
     torch = _L.require_gpu()
     rstd = torch.as_tensor(stats["r_std"], device=R.device, dtype=torch.float32).view(1, -1, 1, 1)
-    return float(((R / rstd) ** 2).mean())
+    return float(((R / rstd) ** 2)[:, applied_fields(stats)].mean())
 
 
 # ------------------------------------------------------------ coarse solver
 class ClosureSolver:
-    """Coarse MHDSolver with the learned flux correction and the plain-step
-    fallback."""
+    """Coarse MHDSolver with the learned flux correction, its positivity
+    limiter and the plain-step fallback.  Every launch of a corrected step
+    (features, the four layers, limiter and source, the SSP-RK3 stages)
+    runs on the context's stream; the state is read in place."""
 
-    def __init__(self, cfg, model: FluxCNN | None, divb_floor: float = 1.0):
+    def __init__(self, cfg, model: FluxCNN | None, divb_floor: float = 1.0, keep: float | None = 0.5,
+                 fields=None):
+        """fields: the network fields (positions in FEAT) whose corrections
+        are applied, None for the model's own (all unless it was trained
+        on fewer); the others keep the plain solver's fluxes."""
         from solver import MHDSolver
 
         self.s = MHDSolver(cfg)
         self.model = model
         self.divb_floor = divb_floor
+        self.keep = keep
+        if fields is None and model is not None:
+            fields = applied_fields(model.stats)
+        self.mask = np.ones(N_IN, dtype=np.float32)
+        if fields is not None:
+            self.mask[:] = 0.0
+            self.mask[list(fields)] = 1.0
         self.fallbacks = 0
-        self.src = None
+        self.limited = 0.0           # summed fraction of cells with theta < 1
+        self._bufs = None
 
     def check(self):
         out = (C.c_double * 4)()
         _L.check(self.s.ctx.lib.amhd_state_check(self.s.ctx.h, out), "amhd_state_check")
         return tuple(out)
 
-    def set_source(self):
+    def _alloc(self):
         torch = _L.require_gpu()
-        cfg = self.s.cfg
-        U = self.s.ctx.download()[:, 2:-2, 2:-2].unsqueeze(0)
-        with torch.no_grad():
-            out = self.model.forward(features(U, self.model.stats))
-            fs = torch.as_tensor(np.concatenate([self.model.stats["f_scale"]] * 2), device=U.device,
-                                 dtype=torch.float32)
-            S = source_from_flux(out[..., :N_OUT] * fs, cfg.dx, cfg.dy)[0]
-        src = torch.zeros(10, cfg.nx, cfg.ny, device=U.device, dtype=torch.float32)
-        src[FEAT] = S.permute(2, 0, 1)
-        self.src = src.contiguous()
-        torch.cuda.current_stream().synchronize()
-        _L.check(self.s.ctx.lib.amhd_set_source(self.s.ctx.h, _p(self.src)), "amhd_set_source")
+        cfg, ctx = self.s.cfg, self.s.ctx
+        nx, ny = cfg.nx, cfg.ny
+        st = C.c_void_p()
+        _L.check(ctx.lib.amhd_get_stream(ctx.h, C.byref(st)), "amhd_get_stream")
+        self.stream = st
+        self.x = torch.empty(1, nx, ny, 16, device="cuda", dtype=torch.bfloat16)
+        self.outs = FluxCNN.alloc_outputs(1, nx, ny)
+        self.theta = torch.ones(2 * nx * ny, device="cuda", dtype=torch.float32)   # theta, theta_mix
+        self._bufs = True
+        torch.cuda.synchronize()
 
-    def clear_source(self):
-        _L.check(self.s.ctx.lib.amhd_set_source(self.s.ctx.h, None), "amhd_set_source")
+    def _state_ptr(self) -> int:
+        ctx = self.s.ctx
+        state = C.c_void_p()
+        _L.check(ctx.lib.amhd_state_ptr(ctx.h, C.byref(state)), "amhd_state_ptr")
+        return state.value
 
-    def run(self, t_end: float, max_steps: int = 10 ** 7):
+    def predict(self):
+        """The network's face corrections for the current (pre-step) state,
+        left in self.outs[-1] (asynchronous on the context stream)."""
+        if self._bufs is None:
+            self._alloc()
+        cfg, ctx = self.s.cfg, self.s.ctx
+        x = features([self._state_ptr()], ctx.dtype, FEAT, cfg.nx, cfg.ny, self.model.stats, self.x, self.stream)
+        return self.model.forward(x, stream=self.stream, bufs=self.outs)
+
+    def correct(self, dt: float):
+        """U += dt * S of the predicted corrections on the current (post-step)
+        state, limited against that state (asynchronous)."""
+        cfg, ctx, st = self.s.cfg, self.s.ctx, self.model.stats
+        u = (C.c_void_p * 1)(self._state_ptr())
+        lim = self.keep is not None
+        _L.check(ctx.lib.amhd_closure_source(_p(self.outs[-1]), u, u, ctx.dtype, _int9(FEAT), 1, cfg.nx, cfg.ny,
+                                             _f9(st["f_scale"] * self.mask), float(cfg.dx), float(cfg.dy), float(dt),
+                                             float(cfg.gamma), float(self.keep) if lim else 0.0,
+                                             _p(self.theta) if lim else None, 1, self.stream),
+                 "amhd_closure_source")
+
+    def begin(self):
+        """Upload an assigned state and take the guard's reference |divB|."""
+        self.s._sync_state()
+        self.div0 = self.check()[2]
+
+    def step(self, dt: float, on_plain=None, track_limiter: bool = False) -> bool:
+        """One coarse step of dt (cfg.glm_ch from a preceding compute_dt): the
+        plain SSP-RK3 step followed by U += dt * S(U_pre) -- the form the
+        network is trained on, r = (U_b - RK3(U_a)) / dt -- then the guard;
+        a failing step is rewound and taken again without the correction.
+        on_plain() runs between the plain step and the correction.  Returns
+        whether the correction was kept."""
         s = self.s
-        s._sync_state()
-        div0 = self.check()[2]
+        s.dt = dt
+        if self.model is None:
+            s.step_ssprk3()
+            return True
+        self.predict()
+        s.step_ssprk3()
+        if on_plain is not None:
+            on_plain()
+        self.correct(dt)
+        rho_min, p_min, divb, nonfinite = self.check()
+        ok = rho_min > 0 and p_min > 0 and nonfinite == 0 and divb <= 1.5 * self.div0 + self.divb_floor
+        if track_limiter and self.keep is not None:
+            self.limited += float((self.theta[:cfg_cells(s)] < 1).float().mean())
+        if not ok:
+            _L.check(s.ctx.lib.amhd_rewind(s.ctx.h), "amhd_rewind")
+            s.step_ssprk3()
+            self.fallbacks += 1
+        self.div0 = self.check()[2]
+        return ok
+
+    def run(self, t_end: float, max_steps: int = 10 ** 7, track_limiter: bool = False):
+        """Coarse steps at the CFL time step until t_end."""
+        s = self.s
+        self.begin()
         while s.t < t_end and s.step < max_steps:
             dt = min(s.compute_dt(), t_end - s.t)
             if dt <= 1e-16:
                 break
-            s.dt = dt
-            if self.model is not None:
-                self.set_source()
-                s.step_ssprk3()
-                rho_min, p_min, divb, nonfinite = self.check()
-                ok = rho_min > 0 and p_min > 0 and nonfinite == 0 and divb <= 1.5 * div0 + self.divb_floor
-                self.clear_source()
-                if not ok:
-                    _L.check(s.ctx.lib.amhd_rewind(s.ctx.h), "amhd_rewind")
-                    s.step_ssprk3()
-                    self.fallbacks += 1
-                div0 = self.check()[2]
-            else:
-                s.step_ssprk3()
+            self.step(dt, track_limiter=track_limiter)
             s.t += dt
             s.step += 1
         return s
 
 
+def cfg_cells(s) -> int:
+    return s.cfg.nx * s.cfg.ny
+
+
 # ------------------------------------------------------------------- judge
-def judge(seeds, model, steps: int = 2000, nx: int = 2048, ny: int = 1024, log=None):
-    """Fine run (the judge) vs plain coarse and closure-corrected coarse runs
-    from the restricted initial condition to the fine run's final time."""
+VARIANTS = {
+    "closure": dict(keep=0.5, fields=None),          # the model's own fields
+    "all": dict(keep=0.5, fields=list(range(N_IN))),
+    "nolimit": dict(keep=None, fields=None),
+    "mix": dict(keep=0.5, fields=[RC]),
+    "mass": dict(keep=0.5, fields=[0, RC]),
+    "hydro": dict(keep=0.5, fields=[0, 1, 2, 3, 7, RC]),
+}
+
+
+def judge(seeds, model, steps: int = 2000, nx: int = 2048, ny: int = 1024, log=None, variants=("closure",)):
+    """Fine run (the judge) vs the plain coarse solver and closure variants
+    (VARIANTS: limiter on/off, which corrections apply) from the restricted
+    initial condition to the fine run's final time.  Errors are taken
+    against the restricted fine state (the coarse grid's best attainable
+    answer): the integral mixing width of the restricted state (mw_err_*)
+    and the L1 density error (l1rho_*); the fine grid's own mixing width is
+    reported as mw_fine."""
     torch = _L.require_gpu()
     from amhd_diag import mixing_width
 
@@ -451,11 +662,14 @@ def judge(seeds, model, steps: int = 2000, nx: int = 2048, ny: int = 1024, log=N
         Uf = s.ctx.download()
         t_end = s.t
         mw_f = mixing_width(Uf.cpu().numpy(), s.cfg)
-        rho_f = restrict(Uf)[0]
         ccfg = coarse_config(s.cfg)
-        rec = {"seed": seed, "t_end": t_end, "mw_fine": mw_f, "fine_wall_s": fine_wall}
-        for tag, m in (("plain", None), ("closure", model)):
-            cs = ClosureSolver(ccfg, m)
+        Ufr = restrict(Uf)
+        rho_f = Ufr[0]
+        mw_fr = mixing_width(ghosted(Ufr).cpu().numpy(), ccfg)
+        rec = {"seed": seed, "t_end": t_end, "mw_fine": mw_f, "mw_fine_restricted": mw_fr, "fine_wall_s": fine_wall}
+        runs = [("plain", None, {})] + [(v, model, VARIANTS[v]) for v in variants]
+        for tag, m, kw in runs:
+            cs = ClosureSolver(ccfg, m, **kw)
             cs.s.U = ghosted(fine["U0"])
             torch.cuda.synchronize()
             t1 = time.perf_counter()
@@ -465,14 +679,23 @@ def judge(seeds, model, steps: int = 2000, nx: int = 2048, ny: int = 1024, log=N
             Uc = cs.s.ctx.download()
             rho_c = Uc[0, 2:-2, 2:-2]
             rec[f"mw_{tag}"] = mixing_width(Uc.cpu().numpy(), ccfg)
+            rec[f"mw_err_{tag}"] = abs(rec[f"mw_{tag}"] - mw_fr) / max(abs(mw_fr), 1e-30)
             rec[f"l1rho_{tag}"] = float((rho_c - rho_f).abs().mean() / rho_f.abs().mean())
             rec[f"steps_{tag}"] = cs.s.step
             rec[f"wall_{tag}_s"] = wall
             rec[f"fallbacks_{tag}"] = cs.fallbacks
-        for tag in ("plain", "closure"):
-            rec[f"mw_err_{tag}"] = abs(rec[f"mw_{tag}"] - mw_f) / max(abs(mw_f), 1e-30)
         rows.append(rec)
         if log:
             log(f"[closure] judge seed {seed}: " + ", ".join(f"{k}={v:.4g}" if isinstance(v, float) else f"{k}={v}"
                                                              for k, v in rec.items()))
     return rows
+
+
+def summarize(rows, tags) -> dict:
+    """Means over the judged seeds."""
+    out = {}
+    for t in tags:
+        for k in ("mw_err", "l1rho", "fallbacks", "steps", "wall"):
+            key = f"wall_{t}_s" if k == "wall" else f"{k}_{t}"
+            out[key + "_mean"] = sum(r[key] for r in rows) / len(rows)
+    return out

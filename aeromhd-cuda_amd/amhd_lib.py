@@ -50,10 +50,10 @@ SIGNATURES = {
     "amhd_reset_boundary_energy": (_i, [_vp, _d]),
     "amhd_compute_rhs": (_i, [_vp, _vp, _vp, _d]),
     "amhd_global_sums": (_i, [_vp, _dp]),
-    "amhd_set_source": (_i, [_vp, _vp]),
     "amhd_rewind": (_i, [_vp]),
     "amhd_state_check": (_i, [_vp, _dp]),
     "amhd_state_ptr": (_i, [_vp, C.POINTER(_vp)]),
+    "amhd_get_stream": (_i, [_vp, C.POINTER(_vp)]),
     "amhd_cons_to_prim": (_i, [_vp, _vp, _l, _d, _vp]),
     "amhd_prim_to_cons": (_i, [_vp, _vp, _l, _d, _vp]),
     "amhd_muscl": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp]),
@@ -73,6 +73,10 @@ SIGNATURES = {
     "amhd_conv3x3_bwd_data": (_i, [_vp, _i, _i, _i, _i, _vp, _i, _vp, _vp, _vp]),
     "amhd_conv3x3_bwd_weight": (_i, [_vp, _vp, _i, _i, _i, _i, _i, _vp, _vp, _vp, _i, _vp]),
     "amhd_mfma_probe": (_i, [_vp, _vp, _vp, _vp]),
+    "amhd_closure_features": (_i, [C.POINTER(_vp), _i, C.POINTER(_i), _i, _i, _i, C.POINTER(C.c_float),
+                                   C.POINTER(C.c_float), _vp, _vp]),
+    "amhd_closure_source": (_i, [_vp, C.POINTER(_vp), C.POINTER(_vp), _i, C.POINTER(_i), _i, _i, _i,
+                                 C.POINTER(C.c_float), _d, _d, _d, _d, _d, _vp, _i, _vp]),
 }
 
 _lib = None

@@ -93,7 +93,6 @@ struct Ctx {
     StepCtl* ctl = nullptr;
     hipStream_t stream = nullptr;
     hipGraphExec_t graph[3] = {nullptr, nullptr, nullptr};
-    const void* src = nullptr;   // optional per-cell source (interior layout, context precision)
 };
 
 size_t esize(int dtype) { return dtype == AMHD_F32 ? sizeof(float) : sizeof(double); }
@@ -124,7 +123,6 @@ StageParams<real> stage_args(const Ctx* c, int stage, const real* u0, const real
     a.ctl = nullptr;
     a.part_dt = stage == 3 ? c->part_dt : nullptr;
     a.part_bf = stage == 1 ? c->part_bf : nullptr;
-    a.src = (const real*)c->src;
     return a;
 }
 
@@ -545,15 +543,6 @@ int amhd_compute_rhs(void* ctx, double* U, double* R, double ch) {
     return 0;
 }
 
-int amhd_set_source(void* ctx, const void* src) {
-    if (!ctx) return fail(-1, "amhd_set_source: null context");
-    Ctx* c = (Ctx*)ctx;
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    if (c->src != src) drop_graphs(c);
-    c->src = src;
-    return 0;
-}
-
 int amhd_rewind(void* ctx) {
     if (!ctx) return fail(-1, "amhd_rewind: null context");
     Ctx* c = (Ctx*)ctx;
@@ -590,6 +579,13 @@ int amhd_state_ptr(void* ctx, void** state) {
     Ctx* c = (Ctx*)ctx;
     HIP_TRY(hipStreamSynchronize(c->stream));
     *state = c->buf[c->cur];
+    c->part_valid = false;   // the caller may edit the state: recompute the CFL partials
+    return 0;
+}
+
+int amhd_get_stream(void* ctx, void** stream) {
+    if (!ctx || !stream) return fail(-1, "amhd_get_stream: null argument");
+    *stream = (void*)((Ctx*)ctx)->stream;
     return 0;
 }
 

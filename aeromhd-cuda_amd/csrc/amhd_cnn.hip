@@ -275,6 +275,262 @@ __global__ void mfma_probe_kernel(const float* __restrict__ A, const float* __re
 
 bool ok_c(int c) { return c == 16 || c == 32; }
 
+// ------------------------------------------- closure inference on the state
+// The network sees NF = 9 conserved fields and returns, per cell, the
+// correction of the 9 x-face fluxes at i-1/2 (channels 0..8) and of the 9
+// y-face fluxes at j-1/2 (channels 9..17) in units of fscale.
+constexpr int NF = 9;
+constexpr int CHUNK = 64;     // instances per launch (pointers travel as kernel arguments)
+
+struct ConstPtrs { const void* p[CHUNK]; };
+struct MutPtrs { void* p[CHUNK]; };
+struct Feat {
+    int field[NF];            // plane of feature f in the instance's array
+    float mean[NF], std[NF];
+};
+struct FluxScale { float s[NF]; int field[NF]; };
+
+// x[(m, i, j)][f] = bf16((U_m[field[f]][i, j] - mean[f]) / std[f]), f < 9;
+// channels 9..15 zero.
+template <class real>
+__global__ __launch_bounds__(256) void closure_features_kernel(ConstPtrs st, Feat ft, int n, long plane,
+                                                               u16* __restrict__ x) {
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (long)n * plane) return;
+    const int m = (int)(k / plane);
+    const long c = k - (long)m * plane;
+    const real* u = (const real*)st.p[m];
+    Frag lo, hi;
+    lo.q = make_uint4(0, 0, 0, 0);
+    hi.q = make_uint4(0, 0, 0, 0);
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+        const u16 h = f2bf((float(u[ft.field[f] * plane + c]) - ft.mean[f]) / ft.std[f]);
+        if (f < 8) lo.h[f] = h; else hi.h[f - 8] = h;
+    }
+    uint4* o = (uint4*)(x + k * 16);
+    o[0] = lo.q;
+    o[1] = hi.q;
+}
+
+// correction of field f through the four faces of cell (m, i, j), each
+// scaled by its face limiter; x faces on the domain boundary carry none.
+struct FaceCorr {
+    float xl[NF], xr[NF], yl[NF], yr[NF];
+};
+__device__ __forceinline__ void load_faces(const float* __restrict__ dF, long p, int i, int j, int nx, int ny,
+                                           FaceCorr& fc) {
+    const long pr = p + ny;
+    const long pu = j + 1 < ny ? p + 1 : p + 1 - ny;
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+        fc.xl[f] = i > 0 ? dF[p * 32 + f] : 0.f;
+        fc.xr[f] = i + 1 < nx ? dF[pr * 32 + f] : 0.f;
+        fc.yl[f] = dF[p * 32 + NF + f];
+        fc.yr[f] = dF[pu * 32 + NF + f];
+    }
+}
+
+// Per-cell limiter.  The correction of cell c is a sum over its four faces,
+// D = sum_f w_f D_f, with face weights w_f = min(theta_left, theta_right).
+// theta_c is the smallest over the faces of the largest t in
+// {1, 1/2, ..., 1/32, 0} for which U + 4 t D_f alone is admissible
+// (rho > 0, rho >= keep * rho(U), p >= keep * p(U); the bounds are rho(U),
+// p(U) themselves where those are not positive).  The admissible set is
+// convex (rho linear, p concave in U) and contains U, so for any weights
+// 0 <= w_f <= theta_c, U + D = (1/4) sum_f (U + 4 w_f D_f) is admissible:
+// the limited correction keeps the bound in every cell.
+template <class real>
+__global__ __launch_bounds__(256) void closure_theta_kernel(const float* __restrict__ dF, ConstPtrs st, FluxScale fs,
+                                                            int n, int nx, int ny, double dtdx, double dtdy,
+                                                            double gamma, double keep, float* __restrict__ theta) {
+    const long plane = (long)nx * ny;
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (long)n * plane) return;
+    const int m = (int)(k / plane);
+    const long c = k - (long)m * plane;
+    const int i = (int)(c / ny), j = (int)(c - (long)i * ny);
+    FaceCorr fc;
+    load_faces(dF, k, i, j, nx, ny, fc);
+    const real* u = (const real*)st.p[m];
+    double u0[10];
+#pragma unroll
+    for (int v = 0; v < 10; ++v) u0[v] = double(u[v * plane + c]);
+    auto pres = [&](const double* w) {
+        return (gamma - 1.0) * (w[7] - 0.5 * (w[1] * w[1] + w[2] * w[2] + w[3] * w[3]) / w[0] -
+                                0.5 * (w[4] * w[4] + w[5] * w[5] + w[6] * w[6]));
+    };
+    const double rmin = u0[0] > 0.0 ? keep * u0[0] : u0[0];
+    const double p0 = pres(u0);
+    const double pmin = p0 > 0.0 ? keep * p0 : p0;
+    // largest grid t with U + 4 t D_f admissible; D_f = sign * scale * face[f] * fscale[f]
+    auto largest = [&](const float* face, double sc) -> double {
+        double t = 1.0;
+        for (int s = 0; s < 6; ++s, t *= 0.5) {
+            double w[10];
+#pragma unroll
+            for (int v = 0; v < 10; ++v) w[v] = u0[v];
+#pragma unroll
+            for (int f = 0; f < NF; ++f) w[fs.field[f]] += 4.0 * t * sc * (double)face[f] * fs.s[f];
+            if (w[0] > 0.0 && w[0] >= rmin && pres(w) >= pmin) return t;
+        }
+        return 0.0;
+    };
+    double th = largest(fc.xl, dtdx);              // S gains +dF(i-1/2) / dx
+    th = fmin(th, largest(fc.xr, -dtdx));          //          -dF(i+1/2) / dx
+    th = fmin(th, largest(fc.yl, dtdy));
+    th = fmin(th, largest(fc.yr, -dtdy));
+    theta[k] = (float)th;
+}
+
+// Position of rho and rhoC among the network fields (the host checks
+// fields[0] == 0 and fields[8] == 9).  The rhoC face flux is rebuilt as
+// donor-cell transport of the mass flux plus the network's mixing flux G:
+// F_rhoC = C_donor * F_rho + G, the donor being the cell upwind of F_rho.
+constexpr int FRHO = 0, FRC = 8;
+
+template <class real>
+__device__ __forceinline__ double conc(const real* __restrict__ u, long plane, long c) {
+    const double r = double(u[c]);
+    return r > 0.0 ? double(u[9 * plane + c]) / r : 0.0;
+}
+
+// The cell's four face weights: min of theta over the face's two cells
+// (x faces on the domain boundary 0; 1 everywhere without a limiter).
+struct FaceW { float xl, xr, yl, yr; };
+__device__ __forceinline__ FaceW face_weights(const float* __restrict__ th, long k, int i, int j, int nx, int ny) {
+    FaceW w{1.f, 1.f, 1.f, 1.f};
+    if (!th) {
+        w.xl = i > 0 ? 1.f : 0.f;
+        w.xr = i + 1 < nx ? 1.f : 0.f;
+        return w;
+    }
+    const float t0 = th[k];
+    w.xl = i > 0 ? fminf(t0, th[k - ny]) : 0.f;
+    w.xr = i + 1 < nx ? fminf(t0, th[k + ny]) : 0.f;
+    w.yl = fminf(t0, th[j > 0 ? k - 1 : k - 1 + ny]);
+    w.yr = fminf(t0, th[j + 1 < ny ? k + 1 : k + 1 - ny]);
+    return w;
+}
+
+// Concentration of the cell and of its four face neighbours (the cell's
+// own value stands in beyond an x boundary, where the face carries nothing).
+struct Nbr { double c, xl, xr, yl, yr; };
+template <class real>
+__device__ __forceinline__ Nbr neighbours(const real* __restrict__ u, long plane, long c, int i, int j, int nx, int ny) {
+    Nbr q;
+    q.c = conc(u, plane, c);
+    q.xl = i > 0 ? conc(u, plane, c - ny) : q.c;
+    q.xr = i + 1 < nx ? conc(u, plane, c + ny) : q.c;
+    q.yl = conc(u, plane, j > 0 ? c - 1 : c - 1 + ny);
+    q.yr = conc(u, plane, j + 1 < ny ? c + 1 : c + 1 - ny);
+    return q;
+}
+
+// Limited physical face fluxes of cell (i, j): non-mixing fields weighted
+// by w, G by wm, F_rhoC = C_donor * F_rho + G.
+struct Fluxes { double xl[NF], xr[NF], yl[NF], yr[NF]; };
+__device__ __forceinline__ void limited_fluxes(const FaceCorr& fc, const FluxScale& fs, const FaceW& w,
+                                               const FaceW& wm, const Nbr& q, Fluxes& F) {
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+        const double sc = fs.s[f];
+        const bool g = f == FRC;
+        F.xl[f] = (g ? wm.xl : w.xl) * (double)fc.xl[f] * sc;
+        F.xr[f] = (g ? wm.xr : w.xr) * (double)fc.xr[f] * sc;
+        F.yl[f] = (g ? wm.yl : w.yl) * (double)fc.yl[f] * sc;
+        F.yr[f] = (g ? wm.yr : w.yr) * (double)fc.yr[f] * sc;
+    }
+    F.xl[FRC] += (F.xl[FRHO] > 0.0 ? q.xl : q.c) * F.xl[FRHO];
+    F.xr[FRC] += (F.xr[FRHO] > 0.0 ? q.c : q.xr) * F.xr[FRHO];
+    F.yl[FRC] += (F.yl[FRHO] > 0.0 ? q.yl : q.c) * F.yl[FRHO];
+    F.yr[FRC] += (F.yr[FRHO] > 0.0 ? q.c : q.yr) * F.yr[FRHO];
+}
+
+// Limiter of the mixing flux: with the mass flux limited by theta and its
+// donor-cell rhoC transport applied (rho1, rc1), the smallest over the
+// faces of the largest t in {1, ..., 1/32, 0} for which rc1 + 4 t G_face
+// alone stays within [C_lo rho1, C_hi rho1], C_lo / C_hi the min / max of C
+// over the cell and its face neighbours.  The set is convex and, by the
+// donor-cell form, holds rc1 itself, so any face weights up to theta_mix
+// keep C within the bounds.
+template <class real>
+__global__ __launch_bounds__(256) void closure_theta_mix_kernel(const float* __restrict__ dF, ConstPtrs st,
+                                                                FluxScale fs, int n, int nx, int ny, double dtdx,
+                                                                double dtdy, const float* __restrict__ theta,
+                                                                float* __restrict__ theta_mix) {
+    const long plane = (long)nx * ny;
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (long)n * plane) return;
+    const int m = (int)(k / plane);
+    const long c = k - (long)m * plane;
+    const int i = (int)(c / ny), j = (int)(c - (long)i * ny);
+    FaceCorr fc;
+    load_faces(dF, k, i, j, nx, ny, fc);
+    const real* u = (const real*)st.p[m];
+    const FaceW w = face_weights(theta, k, i, j, nx, ny);
+    const FaceW zero{0.f, 0.f, 0.f, 0.f};
+    const Nbr q = neighbours(u, plane, c, i, j, nx, ny);
+    Fluxes F;
+    limited_fluxes(fc, fs, w, zero, q, F);
+    const double rho1 = double(u[c]) + dtdx * (F.xl[FRHO] - F.xr[FRHO]) + dtdy * (F.yl[FRHO] - F.yr[FRHO]);
+    const double rc1 = double(u[9 * plane + c]) + dtdx * (F.xl[FRC] - F.xr[FRC]) + dtdy * (F.yl[FRC] - F.yr[FRC]);
+    const double clo = fmin(q.c, fmin(fmin(q.xl, q.xr), fmin(q.yl, q.yr)));
+    const double chi = fmax(q.c, fmax(fmax(q.xl, q.xr), fmax(q.yl, q.yr)));
+    const double lo = clo * rho1, hi = chi * rho1, sc = fs.s[FRC];
+    auto largest = [&](double g) -> double {
+        double t = 1.0;
+        for (int s = 0; s < 6; ++s, t *= 0.5) {
+            const double v = rc1 + 4.0 * t * g;
+            if (v >= lo && v <= hi) return t;
+        }
+        return 0.0;
+    };
+    double th = largest(dtdx * (double)fc.xl[FRC] * sc);
+    th = fmin(th, largest(-dtdx * (double)fc.xr[FRC] * sc));
+    th = fmin(th, largest(dtdy * (double)fc.yl[FRC] * sc));
+    th = fmin(th, largest(-dtdy * (double)fc.yr[FRC] * sc));
+    theta_mix[k] = (float)th;
+}
+
+// S[field[f]] = -((F_xr - F_xl) / dx + (F_yr - F_yl) / dy) of the limited
+// fluxes.  ACC = false: out = S, and the planes that no feature names
+// (psi) get 0; ACC = true: out += dt * S on the named planes (out is the
+// state).  theta / theta_mix null: no limiter.
+template <class real, bool ACC>
+__global__ __launch_bounds__(256) void closure_source_kernel(const float* __restrict__ dF, ConstPtrs st,
+                                                             const float* __restrict__ theta,
+                                                             const float* __restrict__ theta_mix, MutPtrs outs,
+                                                             FluxScale fs, int n, int nx, int ny, double rdx,
+                                                             double rdy, double dt) {
+    const long plane = (long)nx * ny;
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (long)n * plane) return;
+    const int m = (int)(k / plane);
+    const long c = k - (long)m * plane;
+    const int i = (int)(c / ny), j = (int)(c - (long)i * ny);
+    FaceCorr fc;
+    load_faces(dF, k, i, j, nx, ny, fc);
+    const Nbr q = neighbours((const real*)st.p[m], plane, c, i, j, nx, ny);
+    Fluxes F;
+    limited_fluxes(fc, fs, face_weights(theta, k, i, j, nx, ny), face_weights(theta_mix, k, i, j, nx, ny), q, F);
+    real* s = (real*)outs.p[m];
+    bool named[10] = {false, false, false, false, false, false, false, false, false, false};
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+        const double v = -((F.xr[f] - F.xl[f]) * rdx + (F.yr[f] - F.yl[f]) * rdy);
+        real& o = s[fs.field[f] * plane + c];
+        if (ACC) o = real(double(o) + dt * v);
+        else o = real(v);
+        named[fs.field[f]] = true;
+    }
+    if (!ACC) {
+#pragma unroll
+        for (int v = 0; v < 10; ++v)
+            if (!named[v]) s[v * plane + c] = real(0);
+    }
+}
+
 }  // namespace
 
 // shared error channel of the library (amhd_api.hip)
@@ -364,6 +620,87 @@ int amhd_mfma_probe(const float* A, const float* B, float* D, void* stream) {
     if (!A || !B || !D) return cfail(-1, "amhd_mfma_probe: null argument");
     mfma_probe_kernel<<<1, 64, 0, (hipStream_t)stream>>>(A, B, D);
     return hipGetLastError() == hipSuccess ? 0 : cfail(-2, "amhd_mfma_probe: launch failed");
+}
+
+int amhd_closure_features(const void* const* states, int dtype, const int* fields, int n, int nx, int ny,
+                          const float* mean, const float* std, void* x, void* stream) {
+    if (!states || !fields || !mean || !std || !x || n <= 0 || nx <= 0 || ny <= 0)
+        return cfail(-1, "amhd_closure_features: bad argument");
+    if (dtype != AMHD_F64 && dtype != AMHD_F32) return cfail(-1, "amhd_closure_features: bad dtype %d", dtype);
+    Feat ft;
+    for (int f = 0; f < NF; ++f) {
+        if (fields[f] < 0 || fields[f] >= 10) return cfail(-1, "amhd_closure_features: field %d out of range", fields[f]);
+        if (!(std[f] > 0.f)) return cfail(-1, "amhd_closure_features: std[%d] must be positive", f);
+        ft.field[f] = fields[f]; ft.mean[f] = mean[f]; ft.std[f] = std[f];
+    }
+    const long plane = (long)nx * ny;
+    hipStream_t s = (hipStream_t)stream;
+    for (int m0 = 0; m0 < n; m0 += CHUNK) {
+        const int cnt = n - m0 < CHUNK ? n - m0 : CHUNK;
+        ConstPtrs st{};
+        for (int m = 0; m < cnt; ++m) {
+            if (!states[m0 + m]) return cfail(-1, "amhd_closure_features: null state %d", m0 + m);
+            st.p[m] = states[m0 + m];
+        }
+        const long tot = (long)cnt * plane;
+        u16* xo = (u16*)x + (long)m0 * plane * 16;
+        const unsigned g = (unsigned)((tot + 255) / 256);
+        if (dtype == AMHD_F64) closure_features_kernel<double><<<g, 256, 0, s>>>(st, ft, cnt, plane, xo);
+        else closure_features_kernel<float><<<g, 256, 0, s>>>(st, ft, cnt, plane, xo);
+        if (hipGetLastError() != hipSuccess) return cfail(-2, "amhd_closure_features: launch failed");
+    }
+    return 0;
+}
+
+int amhd_closure_source(const float* dF, const void* const* states, void* const* srcs, int dtype, const int* fields,
+                        int n, int nx, int ny, const float* fscale, double dx, double dy, double dt, double gamma,
+                        double keep, float* theta, int accumulate, void* stream) {
+    if (!dF || !states || !srcs || !fields || !fscale || n <= 0 || nx <= 0 || ny <= 0 || !(dx > 0) || !(dy > 0))
+        return cfail(-1, "amhd_closure_source: bad argument");
+    if (dtype != AMHD_F64 && dtype != AMHD_F32) return cfail(-1, "amhd_closure_source: bad dtype %d", dtype);
+    if (fields[FRHO] != 0 || fields[FRC] != 9)
+        return cfail(-1, "amhd_closure_source: fields[0] must be rho (0) and fields[8] rhoC (9)");
+    if ((theta || accumulate) && !(dt >= 0)) return cfail(-1, "amhd_closure_source: dt must be >= 0");
+    if (theta && !(keep >= 0 && keep < 1)) return cfail(-1, "amhd_closure_source: keep must be in [0, 1)");
+    FluxScale fs;
+    for (int f = 0; f < NF; ++f) {
+        if (fields[f] < 0 || fields[f] >= 10) return cfail(-1, "amhd_closure_source: field %d out of range", fields[f]);
+        fs.s[f] = fscale[f]; fs.field[f] = fields[f];
+    }
+    const long plane = (long)nx * ny;
+    const long total = (long)n * plane;
+    hipStream_t s = (hipStream_t)stream;
+    for (int m0 = 0; m0 < n; m0 += CHUNK) {
+        const int cnt = n - m0 < CHUNK ? n - m0 : CHUNK;
+        ConstPtrs st{};
+        MutPtrs sp{};
+        for (int m = 0; m < cnt; ++m) {
+            if (!srcs[m0 + m] || !states[m0 + m]) return cfail(-1, "amhd_closure_source: null pointer %d", m0 + m);
+            sp.p[m] = srcs[m0 + m];
+            st.p[m] = states[m0 + m];
+        }
+        const long tot = (long)cnt * plane;
+        const float* d = dF + (long)m0 * plane * 32;
+        float* th = theta ? theta + (long)m0 * plane : nullptr;
+        float* tm = theta ? theta + total + (long)m0 * plane : nullptr;
+        const unsigned g = (unsigned)((tot + 255) / 256);
+        if (th) {
+            if (dtype == AMHD_F64) {
+                closure_theta_kernel<double><<<g, 256, 0, s>>>(d, st, fs, cnt, nx, ny, dt / dx, dt / dy, gamma, keep, th);
+                closure_theta_mix_kernel<double><<<g, 256, 0, s>>>(d, st, fs, cnt, nx, ny, dt / dx, dt / dy, th, tm);
+            } else {
+                closure_theta_kernel<float><<<g, 256, 0, s>>>(d, st, fs, cnt, nx, ny, dt / dx, dt / dy, gamma, keep, th);
+                closure_theta_mix_kernel<float><<<g, 256, 0, s>>>(d, st, fs, cnt, nx, ny, dt / dx, dt / dy, th, tm);
+            }
+            if (hipGetLastError() != hipSuccess) return cfail(-2, "amhd_closure_source: limiter launch failed");
+        }
+#define SRC(R, A) closure_source_kernel<R, A><<<g, 256, 0, s>>>(d, st, th, tm, sp, fs, cnt, nx, ny, 1.0 / dx, 1.0 / dy, dt)
+        if (dtype == AMHD_F64) { if (accumulate) SRC(double, true); else SRC(double, false); }
+        else { if (accumulate) SRC(float, true); else SRC(float, false); }
+#undef SRC
+        if (hipGetLastError() != hipSuccess) return cfail(-2, "amhd_closure_source: launch failed");
+    }
+    return 0;
 }
 
 }  // extern "C"

@@ -54,7 +54,6 @@ template <class real> struct StageParams {
     const StepCtl* ctl;     // device-resident dt/ch/decay
     double* part_dt;        // STAGE 3: [n_waves][4]
     double* part_bf;        // STAGE 1: [n_strips][2] (non-periodic x only)
-    const real* src;        // optional per-cell source added to L(U) (same layout as us), or null
 };
 
 // ---------------------------------------------------------- buffer access
@@ -300,7 +299,6 @@ __device__ __forceinline__ void stage_wave(const StageParams<real>& a, const int
                 if (v == EN) s = -divb * (w0[iVX] * w0[iBX] + w0[iVY] * w0[iBY] + w0[iVZ] * w0[iBZ]);
                 r = r + s;
             }
-            if (a.src) r = r + a.src[(long)v * a.plane + (long)i * a.pitch + sj];
             if constexpr (STAGE == 0) unew[v] = r;
             else if constexpr (STAGE == 1) unew[v] = ucur[v] + dtr * r;
             else if constexpr (STAGE == 2) unew[v] = real(0.75) * ubase[v] + real(0.25) * (ucur[v] + dtr * r);
@@ -793,7 +791,6 @@ __global__ __launch_bounds__(WAVE* WPB) void batch_stage_kernel(const BatchInst*
         a.gamma = d.gamma; a.dx = d.dx; a.dy = d.dy;
         a.dt = 0.0; a.ch = 0.0; a.decay = 1.0;
         a.ctl = d.ctl;
-        a.src = nullptr;
         a.part_dt = STAGE == 3 ? d.part_dt : nullptr;
         a.part_bf = STAGE == 1 ? d.part_bf : nullptr;
         const int wave = task - d.task_base;

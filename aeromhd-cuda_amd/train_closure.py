@@ -1,11 +1,18 @@
 """Train and judge the learned flux closure on the GPU.
 
-    python aeromhd-cuda_amd/train_closure.py --train 200 --test 200:220 \
+    python aeromhd-cuda_amd/train_closure.py --fields 8 --train 200 --test 200:220 \
         --iters 3000 --out aeromhd-cuda_amd/closure_weights.npz --report closure_report.json
+
+(the recipe of the shipped weights: the mixing flux alone, no aggregation
+rounds), and to judge given weights only:
+
+    python aeromhd-cuda_amd/train_closure.py --judge-only aeromhd-cuda_amd/closure_weights.npz
 
 Data: fine 2048x1024 magnetic-sieve runs (FP32, 2000 steps) for seeds
 0..train-1 restricted 8x to 256x128; held-out seeds for the validation
-loss and the fine-grid judge.  The judge also times the plain FP64
+loss and the fine-grid judge.  After the first fit, each aggregation round
+adds pairs taken along the corrected coarse solver's own trajectories
+(rollout_pairs) and trains further on the union.  The judge also times the plain FP64
 fine-grid solve of one seed for the speed comparison."""
 from __future__ import annotations
 
@@ -41,6 +48,25 @@ def fp64_fine_throughput(seed: int, steps: int):
     return {"mcups": 2048 * 1024 * n / wall / 1e6, "wall_s_for_run": wall * steps / n, "steps": steps}
 
 
+def final_judge(rep, model, judge_seeds, variants, args):
+    """Judge on the held-out seeds (with the limiter switched off as an
+    ablation), time the FP64 fine solve, write the report."""
+    import amhd_closure as cl
+
+    rows = cl.judge(judge_seeds, model, steps=args.fine_steps, log=log, variants=variants + ("nolimit",))
+    rep["judge"] = rows
+    rep["fp64_fine"] = fp64_fine_throughput(judge_seeds[0], args.fine_steps)
+    fine_cells = 2048 * 1024 * args.fine_steps
+    tags = ("plain",) + variants + ("nolimit",)
+    summ = cl.summarize(rows, tags)
+    fp64 = rep["fp64_fine"]["mcups"]
+    for t in tags:
+        summ[f"speedup_{t}_vs_fp64_fine"] = fine_cells / summ[f"wall_{t}_s_mean"] / 1e6 / fp64
+    rep["summary"] = summ
+    log(f"[closure] summary {summ}")
+    json.dump(rep, open(args.report, "w"), indent=1, default=float)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--train", type=int, default=200)
@@ -51,6 +77,15 @@ def main():
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--fine-steps", type=int, default=2000)
     ap.add_argument("--judge", default="")
+    ap.add_argument("--dagger-rounds", type=int, default=0)
+    ap.add_argument("--dagger-seeds", default="0:40")
+    ap.add_argument("--dagger-iters", type=int, default=2000)
+    ap.add_argument("--dagger-lr", type=float, default=3e-4)
+    ap.add_argument("--pullback", type=float, default=0.1)
+    ap.add_argument("--quick-judge", type=int, default=4, help="held-out seeds judged after every phase")
+    ap.add_argument("--variants", default="closure")
+    ap.add_argument("--fields", default="8", help="network fields the closure corrects (8 = the mixing flux)")
+    ap.add_argument("--judge-only", default="", help="judge these weights on the test seeds; no training")
     ap.add_argument("--out", default=os.path.join(os.path.dirname(os.path.abspath(__file__)), "closure_weights.npz"))
     ap.add_argument("--report", default="closure_report.json")
     args = ap.parse_args()
@@ -59,27 +94,57 @@ def main():
     a, b = (int(v) for v in args.test.split(":"))
     test_seeds = list(range(a, b))
     judge_seeds = test_seeds if not args.judge else list(range(*(int(v) for v in args.judge.split(":"))))
+    variants = tuple(args.variants.split(","))
+    if args.judge_only:
+        model = cl.FluxCNN.load(args.judge_only)
+        rep = {"weights": args.judge_only, "fields": cl.applied_fields(model.stats)}
+        final_judge(rep, model, judge_seeds, variants, args)
+        return
     t0 = time.time()
-    X, R = cl.generate_pairs(range(args.train), args.pairs, steps=args.fine_steps, log=log)
-    Xt, Rt = cl.generate_pairs(test_seeds, 4, steps=args.fine_steps, log=log)
+    data = cl.generate_pairs(range(args.train), args.pairs, steps=args.fine_steps, log=log)
+    test = cl.generate_pairs(test_seeds, 4, steps=args.fine_steps, log=log)
     t_data = time.time() - t0
     dx, dy = 6.0 / 256, 2.0 / 128
-    log(f"[closure] data: train {tuple(X.shape)}, test {tuple(Xt.shape)}, {t_data:.0f} s")
+    log(f"[closure] data: train {tuple(data[0].shape)}, test {tuple(test[0].shape)}, {t_data:.0f} s")
     t1 = time.time()
-    model, hist = cl.train(X, R, dx, dy, iters=args.iters, batch=args.batch, lr=args.lr, log=log, holdout=(Xt, Rt))
-    t_train = time.time() - t1
+    fields = None if args.fields == "all" else [int(v) for v in args.fields.split(",")]
+    model, hist = cl.train(data, dx, dy, iters=args.iters, batch=args.batch, lr=args.lr, log=log, holdout=test,
+                           fields=fields)
     model.save(args.out)
+    phases = []
+
+    def quick(tag):
+        model.save(args.out.replace(".npz", f".{tag.replace(' ', '')}.npz"))
+        if args.quick_judge <= 0:
+            return
+        rows = cl.judge(test_seeds[:args.quick_judge], model, steps=args.fine_steps, variants=variants)
+        ph = {"phase": tag, **cl.summarize(rows, ("plain",) + variants)}
+        phases.append(ph)
+        log(f"[closure] phase {tag}: " + ", ".join(f"{k}={v:.4g}" if isinstance(v, float) else f"{k}={v}"
+                                                   for k, v in ph.items()))
+
+    quick("fit")
+    da, db = (int(v) for v in args.dagger_seeds.split(":"))
+    for rnd in range(args.dagger_rounds):
+        roll = cl.rollout_pairs(model, range(da, db), steps=args.fine_steps, pullback=args.pullback, log=log)
+        rt = cl.rollout_pairs(model, test_seeds[:2], starts=4, steps=args.fine_steps, pullback=args.pullback)
+        log(f"[closure] aggregation round {rnd}: {roll[0].shape[0]} rollout pairs; "
+            f"holdout rollout loss {cl.eval_loss(model, rt, dx, dy):.4f}")
+        data = cl.join(data, roll)
+        del roll
+        model, h = cl.train(data, dx, dy, iters=args.dagger_iters, batch=args.batch, lr=args.dagger_lr, log=log,
+                            holdout=rt, model=model)
+        hist += [dict(r, round=rnd + 1) for r in h]
+        model.save(args.out)
+        quick(f"aggregation {rnd + 1}")
+    t_train = time.time() - t1
     rep = {"train_seeds": args.train, "pairs_per_run": args.pairs, "test_seeds": test_seeds,
-           "iters": args.iters, "batch": args.batch, "lr": args.lr, "data_s": t_data, "train_s": t_train,
-           "history": hist, "holdout_loss": cl.eval_loss(model, Xt, Rt, dx, dy),
-           "holdout_loss_plain": cl.baseline_loss(Rt, model.stats)}
+           "iters": args.iters, "fields": args.fields, "dagger_rounds": args.dagger_rounds, "dagger_seeds": args.dagger_seeds,
+           "dagger_iters": args.dagger_iters, "dagger_lr": args.dagger_lr, "batch": args.batch, "lr": args.lr, "data_s": t_data, "train_s": t_train,
+           "history": hist, "phases": phases, "pullback": args.pullback, "holdout_loss": cl.eval_loss(model, test, dx, dy),
+           "holdout_loss_plain": cl.baseline_loss(test[1], model.stats)}
     log(f"[closure] holdout loss {rep['holdout_loss']:.4f} vs plain {rep['holdout_loss_plain']:.4f}")
-    rows = cl.judge(judge_seeds, model, steps=args.fine_steps, log=log)
-    rep["judge"] = rows
-    rep["fp64_fine"] = fp64_fine_throughput(judge_seeds[0], args.fine_steps)
-    fine_cells = 2048 * 1024 * args.fine_steps
-    rep["closure_fine_equiv_mcups"] = [fine_cells / r["wall_closure_s"] / 1e6 for r in rows]
-    json.dump(rep, open(args.report, "w"), indent=1, default=float)
+    final_judge(rep, model, judge_seeds, variants, args)
     log(f"[closure] wrote {args.out} and {args.report}")
 
 

@@ -107,18 +107,19 @@ int amhd_compute_rhs(void* ctx, double* U_dev, double* R_dev, double ch);
 int amhd_global_sums(void* ctx, double out[4]);
 
 /* Learned-closure hooks (no counterpart in the reference).
- * amhd_set_source: a per-cell source S (10 planes of nx*ny cells, the
- * context's precision, device memory, or NULL) added to L(U) in every stage
- * of subsequent steps until cleared -- over one SSP-RK3 step it adds dt*S.
  * amhd_rewind: after a host-driven step, make the previous state current
  * again (the step is discarded).
  * amhd_state_check: out = {min rho, min p, max|divB|, non-finite cells}.
  * amhd_state_ptr: device address of the current interior-layout state
- * (valid until the next step). */
-int amhd_set_source(void* ctx, const void* src);
+ * (10 planes of nx*ny cells, the context's precision; valid until the next
+ * step).  The caller may edit the state through it before the next step
+ * (the cached CFL partials are dropped).
+ * amhd_get_stream: the context's stream (for work that must be ordered
+ * with its steps without a host synchronisation). */
 int amhd_rewind(void* ctx);
 int amhd_state_check(void* ctx, double out[4]);
 int amhd_state_ptr(void* ctx, void** state);
+int amhd_get_stream(void* ctx, void** stream);
 
 /* ---- standalone operators of physics.py / solver.py (float64, device
  * pointers, asynchronous on `stream` unless a host result is returned) */
@@ -193,6 +194,37 @@ int amhd_conv3x3_bwd_weight(const void* x, const float* dy, int n, int nx, int n
 /* D (16x16) = A (16x32) * B (32x16), row-major FP32, operands rounded to BF16:
  * one v_mfma_f32_16x16x32_bf16 (checks the lane layout). */
 int amhd_mfma_probe(const float* A, const float* B, float* D, void* stream);
+/* Closure inference glue over n instances of nx*ny cells.  states[m]: device
+ * address of instance m's 10 planes of nx*ny cells (dtype AMHD_F64/AMHD_F32;
+ * the interior layout of amhd_state_ptr); fields[9]: the plane of each of
+ * the 9 network fields; mean, std, fscale: host float[9].
+ * amhd_closure_features: x (n, nx, ny, 16) BF16, channel f < 9 =
+ * (U[fields[f]] - mean[f]) / std[f] in FP32, channels 9..15 zero.
+ * amhd_closure_source: from the network output dF (n, nx, ny, 32) FP32 --
+ * channel f the x-face flux correction at i-1/2, 9 + f the y-face one at
+ * j-1/2, both in units of fscale[f]; fields[0] must be rho (0) and
+ * fields[8] rhoC (9), whose channels carry the mixing flux G -- the
+ * face fluxes F_f = w * dF_f * fscale[f] for f < 8 and
+ * F_rhoC = C_donor * F_rho + wm * G (C = rhoC/rho of states, taken from
+ * the cell upwind of F_rho), and the source
+ * S[fields[f]] = -((F_f(i+1/2) - F_f(i-1/2)) / dx + (F_f(j+1/2) - F_f(j-1/2)) / dy)
+ * (x faces on both x boundaries carry nothing, y is periodic).
+ * accumulate = 0: srcs[m] (10 planes, dtype) = S, other planes 0;
+ * accumulate = 1: srcs[m] += dt * S on the named planes (srcs[m] may be
+ * states[m]: the post-step correction of a state in place).
+ * theta: NULL (w = wm = 1) or 2*n*nx*ny floats of workspace for the face
+ * weights w = min over the face's two cells of theta_c, wm likewise of
+ * theta_mix_c.  theta_c = min over the cell's faces of the largest t in
+ * {1, 1/2, ..., 1/32, 0} for which states + 4 t dt S_face alone keeps rho
+ * and p at or above keep times their values; theta_mix_c likewise keeps C
+ * within the min/max of C over the cell and its face neighbours.  By
+ * convexity every cell of states + dt*S keeps both bounds, and S stays a
+ * flux difference, so it conserves exactly. */
+int amhd_closure_features(const void* const* states, int dtype, const int* fields, int n, int nx, int ny,
+                          const float* mean, const float* std, void* x, void* stream);
+int amhd_closure_source(const float* dF, const void* const* states, void* const* srcs, int dtype, const int* fields,
+                        int n, int nx, int ny, const float* fscale, double dx, double dy, double dt, double gamma,
+                        double keep, float* theta, int accumulate, void* stream);
 
 #ifdef __cplusplus
 }

@@ -12,11 +12,22 @@ on NHWC arrays (n, nx, ny, C), zero padding in x and periodic in y.  With
 round them (activations and weights are BF16 tensors; FP32 gradients are
 rounded when loaded into the MFMA); sums are in float64.
 
+and the inference glue of the same file: the network input (features),
+the donor-cell rhoC flux (mix_fluxes) and the limited source
+(limiter_theta for positivity, limiter_theta_mix for the bounds of the
+concentration, limited_source).
+
 Pins (tests/test_closure_oracle.py): the forward equals
 torch.nn.functional.conv2d on the CPU (float64) with the same padding; the
 backward passes equal torch autograd of that conv2d; the adjoint identity
 <conv(x), y> = <x, conv^T(y)> holds; a finite-difference check of the whole
-closure loss.
+closure loss (with the donor-cell mixing); limiter_theta meets its
+definition face by face (pressure recomputed independently) and gives
+theta = 1 for a vanishing correction; every cell of U0 + dt *
+limited_source keeps the rho / p bound for random face weights below
+theta and keeps C within its neighbourhood's bounds (the convexity
+arguments), where the unlimited correction breaks both; limited_source
+with unit thetas equals source_from_flux and conserves every field.
 """
 from __future__ import annotations
 
@@ -123,34 +134,209 @@ def net_backward(acts, dout, Ws, bf16: bool = True):
     return gW, gb
 
 
-def source_from_flux(dF, dx, dy):
-    """S = -div(dF): channels 0..8 at x faces i-1/2 (zero on both x
-    boundaries), 9..17 at y faces j-1/2 (periodic)."""
+RC = 8   # position of rhoC among the network fields (FEAT)
+
+
+def mix_fluxes(fx, fy, C):
+    """Face fluxes of the 9 fields with the rhoC channel rebuilt: donor-cell
+    transport of the mass flux plus the network's mixing flux,
+    F_rhoC = C_donor * F_rho + G.  fx, fy: (n, nx, ny, 9) at faces i-1/2,
+    j-1/2; C (n, nx, ny); the donor of face i-1/2 is cell i-1 when
+    F_rho > 0, else cell i (likewise in y).  Returns fx, fy, Cx, Cy."""
+    Cx = np.where(fx[..., 0] > 0, np.roll(C, 1, axis=1), C)
+    Cy = np.where(fy[..., 0] > 0, np.roll(C, 1, axis=2), C)
+    fx = fx.copy()
+    fy = fy.copy()
+    fx[..., RC] = fx[..., RC] + Cx * fx[..., 0]
+    fy[..., RC] = fy[..., RC] + Cy * fy[..., 0]
+    return fx, fy, Cx, Cy
+
+
+def divergence(fx, fy, dx, dy):
+    """-(d_x fx + d_y fy) with fx at i-1/2 (zero on both x boundaries) and
+    fy at j-1/2 (periodic)."""
+    fx = fx.copy()
+    fx[:, 0] = 0.0
+    fxe = np.concatenate([fx, np.zeros_like(fx[:, :1])], axis=1)
+    return -(fxe[:, 1:] - fxe[:, :-1]) / dx - (np.roll(fy, -1, axis=2) - fy) / dy
+
+
+def source_from_flux(dF, dx, dy, C):
+    """S = -div of the face corrections dF (n, nx, ny, 18), already in
+    physical units: channels 0..8 at x faces i-1/2, 9..17 at y faces j-1/2,
+    the rhoC channels being the mixing flux G (mix_fluxes).  C: the
+    concentration of the state the correction is added to."""
     n9 = len(FEAT)
-    fx = dF[..., :n9]
-    z = np.zeros_like(fx[:, :1])
-    fx = np.concatenate([z, fx[:, 1:], z], axis=1)
-    fy = dF[..., n9:2 * n9]
-    return -(fx[:, 1:] - fx[:, :-1]) / dx - (np.roll(fy, -1, axis=2) - fy) / dy
+    fx, fy, _, _ = mix_fluxes(dF[..., :n9], dF[..., n9:2 * n9], C)
+    return divergence(fx, fy, dx, dy)
 
 
-def loss(out, R, fscale, rstd, dx, dy):
+def concentration(U):
+    """C = rhoC / rho of states U (n, 10, nx, ny)."""
+    return U[:, 9] / U[:, 0]
+
+
+def loss(out, R, fscale, rstd, dx, dy, C):
     """Normalised MSE of S(out * fscale) against R (n, 9, nx, ny)."""
-    S = source_from_flux(out[..., :18] * np.concatenate([fscale, fscale]), dx, dy)
+    S = source_from_flux(out[..., :18] * np.concatenate([fscale, fscale]), dx, dy, C)
     e = (S - np.transpose(R, (0, 2, 3, 1))) / rstd
     return float(np.mean(e * e))
 
 
-def loss_grad_out(out, R, fscale, rstd, dx, dy):
-    """d loss / d out (adjoint of the divergence, by hand)."""
+def loss_grad_out(out, R, fscale, rstd, dx, dy, C):
+    """d loss / d out (adjoints of the divergence and of the donor-cell
+    mixing, by hand; the donor choice is piecewise constant)."""
     n9 = len(FEAT)
     fs = np.concatenate([fscale, fscale])
-    S = source_from_flux(out[..., :18] * fs, dx, dy)
+    sc = out[..., :18] * fs
+    fx, fy, Cx, Cy = mix_fluxes(sc[..., :n9], sc[..., n9:2 * n9], C)
+    S = divergence(fx, fy, dx, dy)
     g = 2.0 * (S - np.transpose(R, (0, 2, 3, 1))) / rstd ** 2 / S.size
     gx = np.zeros_like(g)                    # d/d fx_i: S_i has +fx_i/dx, S_{i-1} has -fx_i/dx
     gx[:, 1:] = (g[:, 1:] - g[:, :-1]) / dx
     gy = (g - np.roll(g, 1, axis=2)) / dy
+    gx[..., 0] += Cx * gx[..., RC]
+    gy[..., 0] += Cy * gy[..., RC]
     d = np.zeros_like(out)
     d[..., :n9] = gx * fscale
     d[..., n9:2 * n9] = gy * fscale
     return d
+
+
+# --------------------------------------------------------- inference glue
+THETAS = [1.0, 0.5, 0.25, 0.125, 0.0625, 0.03125]
+
+
+def features(U, fields, mean, std):
+    """BF16 network input (n, nx, ny, 16) of states U (n, planes, nx, ny):
+    channel f = (float32(U[fields[f]]) - mean[f]) / std[f] in float32."""
+    U = np.asarray(U)
+    n, _, nx, ny = U.shape
+    out = np.zeros((n, nx, ny, 16), dtype=np.float32)
+    for f, pl in enumerate(fields):
+        v = (U[:, pl].astype(np.float32) - np.float32(mean[f])) / np.float32(std[f])
+        out[..., f] = v.astype(np.float32)
+    return bf16_round(out)
+
+
+def _pressure(U, gamma):
+    ke = 0.5 * (U[:, 1] ** 2 + U[:, 2] ** 2 + U[:, 3] ** 2) / U[:, 0]
+    me = 0.5 * (U[:, 4] ** 2 + U[:, 5] ** 2 + U[:, 6] ** 2)
+    return (gamma - 1.0) * (U[:, 7] - ke - me)
+
+
+def _faces(f, dt, dx, dy, fy=None):
+    """The increments the four faces of each cell bring alone, dt-scaled:
+    (4, n, nx, ny, ...) for i-1/2, i+1/2, j-1/2, j+1/2; f / fy at faces
+    i-1/2 / j-1/2 (x faces on the domain boundary bring nothing)."""
+    xl = f.copy()
+    xl[:, 0] = 0.0
+    xr = np.zeros_like(f)
+    xr[:, :-1] = f[:, 1:]
+    return np.stack([xl * dt / dx, -xr * dt / dx, fy * dt / dy, -np.roll(fy, -1, axis=2) * dt / dy])
+
+
+def admissible(U0, W, gamma, keep):
+    """rho > 0, rho >= keep*rho0 and p >= keep*p0 (the bound is rho0 / p0
+    itself where those are not positive); U0, W (n, 10, nx, ny)."""
+    rho0, p0 = U0[:, 0], _pressure(U0, gamma)
+    rmin = np.where(rho0 > 0, keep * rho0, rho0)
+    pmin = np.where(p0 > 0, keep * p0, p0)
+    return (W[:, 0] > 0) & (W[:, 0] >= rmin) & (_pressure(W, gamma) >= pmin)
+
+
+def _largest(ok_at):
+    """Per cell, the largest t of THETAS with ok_at(t) (else 0)."""
+    out = None
+    done = None
+    for t in THETAS:
+        ok = ok_at(t)
+        if out is None:
+            out = np.zeros(ok.shape)
+            done = np.zeros(ok.shape, dtype=bool)
+        ok = ok & ~done
+        out[ok] = t
+        done |= ok
+    return out
+
+
+def limiter_theta(U0, dF, fscale, dt, dx, dy, gamma, keep):
+    """Per-cell limiter of the non-mixing corrections: the smallest over
+    the cell's four faces of the largest t of THETAS (else 0) for which
+    U0 + 4 t D_face alone is admissible.  U0 (n, 10, nx, ny), dF (n, nx,
+    ny, 32) -> theta (n, nx, ny)."""
+    U0 = np.asarray(U0, dtype=np.float64)
+    n9 = len(FEAT)
+    dF = np.asarray(dF, dtype=np.float64)
+    theta = np.ones(U0[:, 0].shape)
+    for Df in _faces(dF[..., :n9] * fscale, dt, dx, dy, dF[..., n9:2 * n9] * fscale):
+        D = np.zeros_like(U0)
+        D[:, FEAT] = np.transpose(Df, (0, 3, 1, 2))
+        D[:, 9] = 0.0                       # the mixing channel has its own limiter
+        theta = np.minimum(theta, _largest(lambda t: admissible(U0, U0 + 4.0 * t * D, gamma, keep)))
+    return theta
+
+
+def face_weights(theta):
+    """min over each face's two cells: wx at i-1/2 (0 at i = 0), wy at j-1/2."""
+    wx = np.zeros_like(theta)
+    wx[:, 1:] = np.minimum(theta[:, 1:], theta[:, :-1])
+    wy = np.minimum(theta, np.roll(theta, 1, axis=2))
+    return wx, wy
+
+
+def local_bounds(a):
+    """min and max over each cell and its face neighbours (no neighbour
+    beyond the x boundaries, y periodic)."""
+    lo, hi = a.copy(), a.copy()
+    for nb in (np.roll(a, 1, axis=2), np.roll(a, -1, axis=2)):
+        lo, hi = np.minimum(lo, nb), np.maximum(hi, nb)
+    lo[:, 1:] = np.minimum(lo[:, 1:], a[:, :-1])
+    hi[:, 1:] = np.maximum(hi[:, 1:], a[:, :-1])
+    lo[:, :-1] = np.minimum(lo[:, :-1], a[:, 1:])
+    hi[:, :-1] = np.maximum(hi[:, :-1], a[:, 1:])
+    return lo, hi
+
+
+def limiter_theta_mix(U0, dF, theta, fscale, dt, dx, dy):
+    """Per-cell limiter of the mixing flux G: with the mass flux limited by
+    theta's face weights and its donor-cell rhoC transport applied, the
+    smallest over the faces of the largest t of THETAS (else 0) for which
+    the face's G alone, times 4 t, keeps C = rhoC/rho within the min and max
+    of C over the cell and its face neighbours in U0."""
+    U0 = np.asarray(U0, dtype=np.float64)
+    n9 = len(FEAT)
+    dF = np.asarray(dF, dtype=np.float64)
+    C = concentration(U0)
+    wx, wy = face_weights(theta)
+    fx = dF[..., :n9] * fscale
+    fy = dF[..., n9:2 * n9] * fscale
+    mx = np.zeros_like(fx)
+    my = np.zeros_like(fy)
+    mx[..., 0] = wx * fx[..., 0]
+    my[..., 0] = wy * fy[..., 0]
+    mx, my, _, _ = mix_fluxes(mx, my, C)     # rho and donor rhoC parts only
+    Sm = divergence(mx, my, dx, dy)
+    rho1 = U0[:, 0] + dt * Sm[..., 0]
+    rc1 = U0[:, 9] + dt * Sm[..., RC]
+    clo, chi = local_bounds(C)
+    th = np.ones(theta.shape)
+    for G in _faces(fx[..., RC], dt, dx, dy, fy[..., RC]):
+        th = np.minimum(th, _largest(lambda t: (rc1 + 4.0 * t * G >= clo * rho1) & (rc1 + 4.0 * t * G <= chi * rho1)))
+    return th
+
+
+def limited_source(dF, theta, theta_mix, fscale, dx, dy, C):
+    """-div of the face corrections, the non-mixing channels scaled by
+    theta's face weights and the mixing flux G by theta_mix's; the rhoC
+    flux is C_donor * (weighted F_rho) + weighted G.  Returns (n, nx, ny, 9)."""
+    n9 = len(FEAT)
+    dF = np.asarray(dF, dtype=np.float64)
+    wx, wy = face_weights(theta)
+    gx, gy = face_weights(theta_mix)
+    fx = dF[..., :n9] * fscale * wx[..., None]
+    fy = dF[..., n9:2 * n9] * fscale * wy[..., None]
+    fx[..., RC] = dF[..., RC] * fscale[RC] * gx
+    fy[..., RC] = dF[..., n9 + RC] * fscale[RC] * gy
+    fx, fy, _, _ = mix_fluxes(fx, fy, C)
+    return divergence(fx, fy, dx, dy)

@@ -72,12 +72,13 @@ def test_network_gradient_finite_differences():
         Ws.append(W)
         bs.append(rng.standard_normal(cout) * 0.1)
     R = rng.standard_normal((n, 9, nx, ny))
+    Cc = rng.uniform(0, 1, (n, nx, ny))
     fscale, rstd, dx, dy = rng.uniform(0.5, 2, 9), rng.uniform(0.5, 2, 9), 0.3, 0.2
     out, acts = co.net_forward(x, Ws, bs, bf16=False)
-    gW, gb = co.net_backward(acts, co.loss_grad_out(out, R, fscale, rstd, dx, dy), Ws, bf16=False)
+    gW, gb = co.net_backward(acts, co.loss_grad_out(out, R, fscale, rstd, dx, dy, Cc), Ws, bf16=False)
 
     def L():
-        return co.loss(co.net_forward(x, Ws, bs, bf16=False)[0], R, fscale, rstd, dx, dy)
+        return co.loss(co.net_forward(x, Ws, bs, bf16=False)[0], R, fscale, rstd, dx, dy, Cc)
 
     h = 1e-6
     for li, (cin, cout, _) in enumerate(co.LAYERS):
@@ -96,3 +97,156 @@ def test_network_gradient_finite_differences():
         lm = L()
         bs[li][1] = old
         assert gb[li][1] == pytest.approx((lp - lm) / (2 * h), rel=1e-5, abs=1e-9)
+
+
+def _state(rng, n, nx, ny, gamma):
+    """Random conserved states with rho in [0.5, 2] and p in [0.2, 1]."""
+    U = np.zeros((n, 10, nx, ny))
+    rho = rng.uniform(0.5, 2.0, (n, nx, ny))
+    v = rng.uniform(-1, 1, (3, n, nx, ny))
+    B = rng.uniform(-1, 1, (3, n, nx, ny))
+    p = rng.uniform(0.2, 1.0, (n, nx, ny))
+    U[:, 0] = rho
+    U[:, 1:4] = np.moveaxis(rho * v, 0, 1)
+    U[:, 4:7] = np.moveaxis(B, 0, 1)
+    U[:, 7] = p / (gamma - 1) + 0.5 * rho * (v ** 2).sum(0) + 0.5 * (B ** 2).sum(0)
+    U[:, 9] = rho * rng.uniform(0, 1, (n, nx, ny))
+    return U
+
+
+def _admissible(U0, D, t, gamma, keep):
+    U = U0 + t * D
+    p = lambda A: (gamma - 1) * (A[7] - 0.5 * (A[1] ** 2 + A[2] ** 2 + A[3] ** 2) / A[0]
+                                 - 0.5 * (A[4] ** 2 + A[5] ** 2 + A[6] ** 2))
+    return U[0] > 0 and U[0] >= keep * U0[0] and p(U) >= keep * p(U0)
+
+
+def test_limiter_theta_definition():
+    """theta = min over faces of the largest grid t with U0 + 4 t D_face
+    admissible (checked cell by cell with an independent pressure), and the
+    next grid value up fails for some face; theta = 1 for a vanishing
+    correction."""
+    rng = np.random.default_rng(21)
+    gamma, dx, dy, dt, keep = 5.0 / 3.0, 0.1, 0.08, 0.01, 0.5
+    U0 = _state(rng, 1, 8, 6, gamma)
+    dF = rng.standard_normal((1, 8, 6, 32)) * 0.7
+    fs = rng.uniform(0.5, 2.0, 9)
+    th = co.limiter_theta(U0, dF, fs, dt, dx, dy, gamma, keep)
+    n9 = 9
+    grid = co.THETAS + [0.0]
+    n_lim = 0
+    for i in range(8):
+        for j in range(6):
+            faces = []
+            fx = dF[0, :, :, :n9] * fs
+            fy = dF[0, :, :, n9:18] * fs
+            faces.append(fx[i, j] * dt / dx if i > 0 else np.zeros(9))
+            faces.append(-fx[i + 1, j] * dt / dx if i + 1 < 8 else np.zeros(9))
+            faces.append(fy[i, j] * dt / dy)
+            faces.append(-fy[i, (j + 1) % 6] * dt / dy)
+            t = th[0, i, j]
+            assert t in grid
+            for Df in faces:
+                D = np.zeros(10)
+                D[co.FEAT] = Df
+                assert t == 0.0 or _admissible(U0[0, :, i, j], 4 * t * D, 1.0, gamma, keep)
+            if t < 1.0:
+                n_lim += 1
+                up = grid[grid.index(t) - 1]
+                fails = []
+                for Df in faces:
+                    D = np.zeros(10)
+                    D[co.FEAT] = Df
+                    fails.append(not _admissible(U0[0, :, i, j], 4 * up * D, 1.0, gamma, keep))
+                assert any(fails)
+    assert 0 < n_lim < 48
+    assert np.all(co.limiter_theta(U0, dF * 1e-9, fs, dt, dx, dy, gamma, keep) == 1.0)
+
+
+def test_limited_correction_keeps_the_bound():
+    """Every cell of U0 + dt * S stays admissible when S is built from face
+    weights in [0, min(theta_L, theta_R)] -- the limiter's guarantee --
+    while the unlimited correction breaks the bound somewhere."""
+    rng = np.random.default_rng(23)
+    gamma, dx, dy, dt, keep = 5.0 / 3.0, 0.1, 0.08, 0.02, 0.5
+    U0 = _state(rng, 2, 9, 7, gamma)
+    C = co.concentration(U0)
+    dF = rng.standard_normal((2, 9, 7, 32)) * 4.0
+    fs = rng.uniform(0.5, 2.0, 9)
+    th = co.limiter_theta(U0, dF, fs, dt, dx, dy, gamma, keep)
+    assert np.any(th < 1)
+    S0 = co.source_from_flux(dF[..., :18] * np.concatenate([fs, fs]), dx, dy, C)
+    W0 = U0.copy()
+    W0[:, co.FEAT] += dt * np.transpose(S0, (0, 3, 1, 2))
+    assert not np.all(co.admissible(U0, W0, gamma, keep))
+    for trial in range(3):
+        # random face weights below the face limiter: scale dF face by face
+        w = rng.uniform(0, 1, (2, 9, 7, 32)).astype(np.float64)
+        S = co.limited_source(dF * w, th, np.ones_like(th), fs, dx, dy, C)
+        W = U0.copy()
+        W[:, co.FEAT] += dt * np.transpose(S, (0, 3, 1, 2))
+        assert np.all(co.admissible(U0, W, gamma, keep)), trial
+
+
+def test_mixing_limiter_keeps_concentration_bounds():
+    """C of U0 + dt * limited_source stays within the min / max of C over
+    each cell and its face neighbours in U0 (a discrete maximum principle);
+    without the mixing limiter it does not."""
+    rng = np.random.default_rng(24)
+    gamma, dx, dy, dt, keep = 5.0 / 3.0, 0.1, 0.08, 0.02, 0.5
+    U0 = _state(rng, 2, 10, 8, gamma)
+    U0[:, 9] = U0[:, 0] * (rng.uniform(0, 1, (2, 10, 8)) > 0.5)     # pure cells, C in {0, 1}
+    U0[:, 9, 3:5] = U0[:, 0, 3:5] * 0.4                              # and a mixed band
+    C = co.concentration(U0)
+    dF = rng.standard_normal((2, 10, 8, 32)) * 1.5
+    fs = rng.uniform(0.5, 2.0, 9)
+    th = co.limiter_theta(U0, dF, fs, dt, dx, dy, gamma, keep)
+    tm = co.limiter_theta_mix(U0, dF, th, fs, dt, dx, dy)
+    assert np.any(tm < 1) and np.any(tm > 0)
+    lo, hi = co.local_bounds(C)
+
+    def c_after(theta_mix):
+        S = co.limited_source(dF, th, theta_mix, fs, dx, dy, C)
+        W = U0.copy()
+        W[:, co.FEAT] += dt * np.transpose(S, (0, 3, 1, 2))
+        return W[:, 9] / W[:, 0]
+
+    c1 = c_after(tm)
+    assert np.all(c1 >= lo - 1e-12) and np.all(c1 <= hi + 1e-12)
+    c0 = c_after(np.ones_like(tm))
+    assert np.any(c0 < lo - 1e-6) or np.any(c0 > hi + 1e-6)
+    # the donor-cell part alone (no mixing flux) keeps the bounds for any theta
+    c2 = c_after(np.zeros_like(tm))
+    assert np.all(c2 >= lo - 1e-12) and np.all(c2 <= hi + 1e-12)
+
+
+def test_limited_source_reduces_and_conserves():
+    rng = np.random.default_rng(22)
+    dF = rng.standard_normal((2, 7, 5, 32))
+    fs = rng.uniform(0.5, 2.0, 9)
+    C = rng.uniform(0, 1, (2, 7, 5))
+    one = np.ones((2, 7, 5))
+    S1 = co.limited_source(dF, one, one, fs, 0.3, 0.2, C)
+    np.testing.assert_allclose(S1, co.source_from_flux(dF[..., :18] * np.concatenate([fs, fs]), 0.3, 0.2, C),
+                               rtol=1e-13, atol=1e-13)
+    th = rng.choice(co.THETAS + [0.0], (2, 7, 5))
+    tm = rng.choice(co.THETAS + [0.0], (2, 7, 5))
+    S = co.limited_source(dF, th, tm, fs, 0.3, 0.2, C)
+    np.testing.assert_allclose(S.sum(axis=(1, 2)), 0.0, atol=1e-12)
+    zero = np.zeros((2, 7, 5))
+    assert np.all(co.limited_source(dF, zero, zero, fs, 0.3, 0.2, C) == 0.0)
+
+
+def test_donor_cell_flux():
+    """F_rhoC = C_donor F_rho + G with the donor upstream of the mass flux."""
+    fx = np.zeros((1, 3, 2, 9))
+    fy = np.zeros((1, 3, 2, 9))
+    C = np.array([[[0.1, 0.2], [0.3, 0.4], [0.5, 0.6]]])
+    fx[0, 1, 0, 0], fx[0, 2, 1, 0] = 2.0, -3.0       # face 1-1/2 from cell 0; face 2-1/2 from cell 2
+    fx[0, 1, 0, 8] = 0.25
+    fy[0, 1, 0, 0], fy[0, 2, 1, 0] = 1.0, -1.0       # face j=0-1/2 from cell j=1 (periodic); face 1-1/2 from j=1
+    ox, oy, _, _ = co.mix_fluxes(fx, fy, C)
+    assert ox[0, 1, 0, 8] == 0.25 + 0.1 * 2.0
+    assert ox[0, 2, 1, 8] == 0.6 * -3.0
+    assert oy[0, 1, 0, 8] == 0.4 * 1.0
+    assert oy[0, 2, 1, 8] == 0.6 * -1.0

@@ -88,12 +88,14 @@ def test_network_gradients_vs_oracle(torch_gpu):
     model = cl.FluxCNN(seed=3)
     X = torch.tensor(rng.standard_normal((n, 9, nx, ny)), device="cuda", dtype=torch.float32)
     R = torch.tensor(rng.standard_normal((n, 9, nx, ny)), device="cuda", dtype=torch.float32)
+    Cp = torch.tensor(rng.uniform(0, 1, (n, nx, ny)), device="cuda", dtype=torch.float32)
+    Cph = Cp.cpu().numpy().astype(np.float64)
     model.stats = {"in_mean": np.zeros(9, np.float32), "in_std": np.ones(9, np.float32),
                    "r_std": rng.uniform(0.5, 2, 9).astype(np.float32),
                    "f_scale": rng.uniform(0.5, 2, 9).astype(np.float32)}
     st = model.stats
     dx, dy = 0.25, 0.2
-    loss, dout, acts = cl.loss_and_grad(model, X, R, st, dx, dy)
+    loss, dout, acts = cl.loss_and_grad(model, X, R, Cp, st, dx, dy)
     gW, gb = model.backward(acts, dout, n_split=3)
     Ws = [W.cpu().numpy() for W in model.W]
     bs = [b.cpu().numpy() for b in model.b]
@@ -102,8 +104,9 @@ def test_network_gradients_vs_oracle(torch_gpu):
     out_ref, _ = co.net_forward(gacts[0], Ws, bs)
     assert np.linalg.norm(out_gpu - out_ref) <= 1e-2 * np.linalg.norm(out_ref)
     Rn = R.cpu().numpy().astype(np.float64)
-    assert loss == pytest.approx(co.loss(out_gpu, Rn, st["f_scale"], st["r_std"], dx, dy), rel=1e-5)
-    dref = co.loss_grad_out(out_gpu, Rn, st["f_scale"].astype(np.float64), st["r_std"].astype(np.float64), dx, dy)
+    assert loss == pytest.approx(co.loss(out_gpu, Rn, st["f_scale"], st["r_std"], dx, dy, Cph), rel=1e-5)
+    dref = co.loss_grad_out(out_gpu, Rn, st["f_scale"].astype(np.float64), st["r_std"].astype(np.float64), dx, dy,
+                            Cph)
     np.testing.assert_allclose(dout.cpu().numpy(), dref, rtol=1e-4, atol=1e-9)
     rW, rb = co.net_backward(gacts, dout.cpu().numpy().astype(np.float64), Ws)
     for li in range(len(co.LAYERS)):
@@ -112,16 +115,104 @@ def test_network_gradients_vs_oracle(torch_gpu):
         assert np.linalg.norm(gb[li].cpu().numpy() - rb[li]) <= 1e-3 * np.linalg.norm(rb[li]) + 1e-9, li
 
 
+@pytest.mark.parametrize("dtype", [0, 1])
+def test_closure_glue_vs_oracle(torch_gpu, dtype):
+    """Network input and limited source of n states in place (FP64 and
+    FP32 states): the features bit for bit; theta equal to the oracle's in
+    all but at most one cell (double sums in another order can land on the
+    other side of a bound); the source equal to the oracle's from the GPU's
+    theta (FP32: 1e-6 of max|S|, FP64: 1e-12); without the limiter, the
+    plain divergence; every field conserved; in place, U + dt*S with rho
+    and p of every cell at or above half their values."""
+    torch = torch_gpu
+    import amhd_closure as cl
+    import amhd_lib as L
+    from test_closure_oracle import _state
+
+    lib = L.load()
+    rng = np.random.default_rng(30 + dtype)
+    gamma, dx, dy, dt, keep = 5.0 / 3.0, 0.1, 0.08, 0.01, 0.5
+    n, nx, ny = 3, 12, 10
+    tdt = torch.float64 if dtype == 0 else torch.float32
+    Ud = torch.tensor(_state(rng, n, nx, ny, gamma), dtype=tdt, device="cuda").contiguous()
+    Uh = Ud.cpu().numpy().astype(np.float64)
+    esz = Ud.element_size()
+    ptrs = [Ud.data_ptr() + m * 10 * nx * ny * esz for m in range(n)]
+    stats = {"in_mean": rng.standard_normal(9).astype(np.float32),
+             "in_std": rng.uniform(0.5, 2, 9).astype(np.float32)}
+    x = torch.empty(n, nx, ny, 16, device="cuda", dtype=torch.bfloat16)
+    cl.features(ptrs, dtype, cl.FEAT, nx, ny, stats, x)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(x.float().cpu().numpy(), co.features(Uh, cl.FEAT, stats["in_mean"], stats["in_std"]))
+
+    dF = torch.tensor(rng.standard_normal((n, nx, ny, 32)) * 0.3, dtype=torch.float32, device="cuda")
+    fs = rng.uniform(0.5, 2, 9).astype(np.float32)
+    theta = torch.empty(2 * n * nx * ny, device="cuda")
+    src = torch.empty(n, 10, nx, ny, device="cuda", dtype=tdt)
+    sp = [src.data_ptr() + m * 10 * nx * ny * esz for m in range(n)]
+    V = C.c_void_p * n
+
+    def run(th, out=sp, acc=0):
+        L.check(lib.amhd_closure_source(_p(dF), V(*ptrs), V(*out), dtype, cl._int9(cl.FEAT), n, nx, ny, cl._f9(fs),
+                                        dx, dy, dt, gamma, keep, _p(th) if th is not None else None, acc,
+                                        L.stream_handle(torch)), "amhd_closure_source")
+        torch.cuda.synchronize()
+        return src.cpu().numpy().astype(np.float64)
+
+    S = run(theta)
+    dFh = dF.cpu().numpy().astype(np.float64)
+    fs64 = fs.astype(np.float64)
+    Ch = co.concentration(Uh)
+    th_ref = co.limiter_theta(Uh, dFh, fs64, dt, dx, dy, gamma, keep)
+    th_all = theta.cpu().numpy().astype(np.float64)
+    th_gpu = th_all[:n * nx * ny].reshape(n, nx, ny)
+    tm_gpu = th_all[n * nx * ny:].reshape(n, nx, ny)
+    assert np.sum(th_gpu != th_ref) <= 1
+    assert 0 < np.sum(th_gpu < 1) < th_gpu.size
+    tm_ref = co.limiter_theta_mix(Uh, dFh, th_gpu, fs64, dt, dx, dy)
+    assert np.sum(tm_gpu != tm_ref) <= 1
+    assert 0 < np.sum(tm_gpu < 1) < tm_gpu.size
+    tol = 1e-6 if dtype == 1 else 1e-12
+    ref = co.limited_source(dFh, th_gpu, tm_gpu, fs64, dx, dy, Ch)
+    scale = np.abs(ref).max()
+    np.testing.assert_allclose(np.transpose(S[:, cl.FEAT], (0, 2, 3, 1)), ref, rtol=0, atol=tol * scale)
+    assert np.all(S[:, 8] == 0)
+    assert np.all(np.abs(S[:, cl.FEAT].sum(axis=(2, 3))) <= 1e-5 * np.abs(S).sum(axis=(2, 3))[:, cl.FEAT] + 1e-9)
+    S1 = run(None)
+    ref1 = co.source_from_flux(dFh[..., :18] * np.concatenate([fs64, fs64]), dx, dy, Ch)
+    np.testing.assert_allclose(np.transpose(S1[:, cl.FEAT], (0, 2, 3, 1)), ref1, rtol=0,
+                               atol=tol * np.abs(ref1).max())
+    # in place on the states: U += dt * S with the limiter taken on U itself
+    run(theta, out=ptrs, acc=1)
+    Unew = Ud.cpu().numpy().astype(np.float64)
+    exp = Uh.copy()
+    exp[:, cl.FEAT] += dt * np.transpose(ref, (0, 3, 1, 2))
+    np.testing.assert_allclose(Unew, exp, rtol=2 * tol, atol=tol * np.abs(Uh).max())
+    assert np.all(Unew[:, 8] == Uh[:, 8])
+    # the limiters' guarantees on the result: rho and p keep half their
+    # values, C stays within its neighbourhood's bounds
+    p0, p1 = co._pressure(Uh, gamma), co._pressure(Unew, gamma)
+    assert np.all(Unew[:, 0] >= keep * Uh[:, 0] * (1 - 10 * tol))
+    assert np.all(p1 >= keep * p0 - 10 * tol * np.abs(Uh[:, 7]))
+    lo, hi = co.local_bounds(Ch)
+    c1 = Unew[:, 9] / Unew[:, 0]
+    assert np.all(c1 >= lo - 10 * tol) and np.all(c1 <= hi + 10 * tol)
+
+
 def test_closure_gain_over_plain_coarse(torch_gpu):
-    """The trained network (closure_weights.npz) against the plain coarse
-    solver on held-out seeds, judged by the fine 2048x1024 solver: lower L1
-    density error; the mixing width is reported."""
+    """The shipped network (closure_weights.npz, which applies the mixing
+    flux alone) against the plain coarse solver on two held-out seeds,
+    judged by the fine 2048x1024 solver restricted to the coarse grid: the
+    mixing-width error at most half the plain solver's, the density (which
+    the mixing flux cannot change) as the plain solver's, no fallbacks."""
     import amhd_closure as cl
 
     path = os.path.join(PKG, "closure_weights.npz")
     assert os.path.exists(path), "closure_weights.npz (trained weights) missing"
     model = cl.FluxCNN.load(path)
+    assert cl.applied_fields(model.stats) == [cl.RC]
     rows = cl.judge([210, 217], model, log=print)
     for r in rows:
-        assert r["l1rho_closure"] < r["l1rho_plain"], r
-        assert r["fallbacks_closure"] < r["steps_closure"], r
+        assert r["mw_err_closure"] < 0.5 * r["mw_err_plain"], r
+        assert r["l1rho_closure"] == pytest.approx(r["l1rho_plain"], rel=1e-6), r
+        assert r["fallbacks_closure"] == 0, r
