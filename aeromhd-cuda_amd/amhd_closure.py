@@ -691,6 +691,27 @@ def judge(seeds, model, steps: int = 2000, nx: int = 2048, ny: int = 1024, log=N
     return rows
 
 
+def fp64_fine_throughput(seed: int, steps: int = 2000, nx: int = 2048, ny: int = 1024) -> dict:
+    """Cell-updates per second of the plain FP64 fine solve of seed's
+    magnetic sieve (device loop, graph replay), the closure's reference."""
+    torch = _L.require_gpu()
+    from amhd_ics import magnetic_sieve
+    from physics import prim_to_cons
+    from solver import MHDSolver
+
+    cfg, W = magnetic_sieve(nx, ny, precision="fp64", seed=seed, max_steps=steps)
+    cfg.t_end = 1e30
+    s = MHDSolver(cfg)
+    s.U = prim_to_cons(W, cfg.gamma)
+    s.advance(5)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    n = s.advance(steps - 5)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return {"mcups": nx * ny * n / wall / 1e6, "wall_s_for_run": wall * steps / n, "steps": steps}
+
+
 def summarize(rows, tags) -> dict:
     """Means over the judged seeds."""
     out = {}

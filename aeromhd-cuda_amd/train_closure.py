@@ -29,25 +29,6 @@ def log(*a):
     print(*a, flush=True)
 
 
-def fp64_fine_throughput(seed: int, steps: int):
-    import torch
-    from amhd_ics import magnetic_sieve
-    from physics import prim_to_cons
-    from solver import MHDSolver
-
-    cfg, W = magnetic_sieve(2048, 1024, precision="fp64", seed=seed, max_steps=steps)
-    cfg.t_end = 1e30
-    s = MHDSolver(cfg)
-    s.U = prim_to_cons(W, cfg.gamma)
-    s.advance(5)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    n = s.advance(steps - 5)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    return {"mcups": 2048 * 1024 * n / wall / 1e6, "wall_s_for_run": wall * steps / n, "steps": steps}
-
-
 def final_judge(rep, model, judge_seeds, variants, args):
     """Judge on the held-out seeds (with the limiter switched off as an
     ablation), time the FP64 fine solve, write the report."""
@@ -55,7 +36,7 @@ def final_judge(rep, model, judge_seeds, variants, args):
 
     rows = cl.judge(judge_seeds, model, steps=args.fine_steps, log=log, variants=variants + ("nolimit",))
     rep["judge"] = rows
-    rep["fp64_fine"] = fp64_fine_throughput(judge_seeds[0], args.fine_steps)
+    rep["fp64_fine"] = cl.fp64_fine_throughput(judge_seeds[0], args.fine_steps)
     fine_cells = 2048 * 1024 * args.fine_steps
     tags = ("plain",) + variants + ("nolimit",)
     summ = cl.summarize(rows, tags)

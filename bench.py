@@ -11,6 +11,9 @@ Prints ONE JSON line: Mcell-updates/s, the roofline of the fused stage
 kernel (algorithmic bytes 8 * 10 * sizeof(real) per cell-update divided by
 the stage kernels' device time, HIP events on the context stream over the
 timed region), and the CPU oracle timed on this host on a bounded sample.
+Secondary records: the 1024x512 RMI in FP64 and FP32, the batched form (64
+instances), and the learned closure's fine-equivalent rate over the FP64
+fine solve with its mixing-width and density errors.
 
   python bench.py [--steps K] [--warmup W] [--workload sieve2048|rmi1024-fp64|rmi1024-fp32]
 """
@@ -23,7 +26,8 @@ import sys
 import time
 
 REPO = os.path.dirname(os.path.abspath(__file__))
-sys.path.insert(0, os.path.join(REPO, "aeromhd-cuda_amd"))
+PKG = os.path.join(REPO, "aeromhd-cuda_amd")
+sys.path.insert(0, PKG)
 
 METRIC = "Mcell-updates/s per RK3 step + % HBM roofline; closure speedup at equal L1 err"
 HBM_PEAK_GBS = 8000.0
@@ -143,6 +147,29 @@ def batch_measure(n: int, steps: int, warmup: int, seed: int, seq_steps: int = 2
             "sequential_value": round(cells * seq_steps / seq_wall / 1e6, 2)}
 
 
+def closure_measure(seed: int = 210, fine_steps: int = 2000):
+    """The learned closure on one held-out magnetic-sieve seed: coarse
+    256x128 FP32 with the shipped weights vs the plain coarse solver, both
+    judged against the fine 2048x1024 run restricted to the coarse grid, and
+    the closure's fine-equivalent cell-update rate over the plain FP64 fine
+    solve's."""
+    import amhd_closure as cl
+
+    model = cl.FluxCNN.load(os.path.join(PKG, "closure_weights.npz"))
+    r = cl.judge([seed], model, steps=fine_steps)[0]
+    fp64 = cl.fp64_fine_throughput(seed, fine_steps)
+    fine_cells = 2048 * 1024 * fine_steps
+    return {"workload": f"learned closure, magnetic sieve seed {seed} (held out): coarse 256x128 FP32 + BF16-MFMA "
+                        "CNN vs the FP64 fine 2048x1024 solve",
+            "value": round(fine_cells / r["wall_closure_s"] / 1e6 / fp64["mcups"], 2),
+            "unit": "x fine-equivalent Mcell-updates/s over FP64 fine",
+            "fp64_fine_mcups": round(fp64["mcups"], 1),
+            "mw_err_closure": round(r["mw_err_closure"], 4), "mw_err_plain": round(r["mw_err_plain"], 4),
+            "l1rho_closure": round(r["l1rho_closure"], 5), "l1rho_plain": round(r["l1rho_plain"], 5),
+            "fallbacks": r["fallbacks_closure"], "coarse_steps": r["steps_closure"],
+            "wall_closure_s": round(r["wall_closure_s"], 4)}
+
+
 def cpu_baseline(seed: int, steps: int, threads: int):
     """The CPU oracle (C restatement of the reference math, FP64, OpenMP
     over x rows) on the 2048 x 1024 sieve: `steps` SSP-RK3 steps with the
@@ -205,6 +232,7 @@ def main():
     ap.add_argument("--copy-calib", action="store_true", help="run a 1 GiB device copy first (PMC scaling)")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-steps", type=int, default=500)
+    ap.add_argument("--no-closure", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -234,6 +262,10 @@ def main():
     if not args.no_secondary and not args.no_batch:
         log("[bench] secondary batch64")
         secondary.append(batch_measure(64, args.batch_steps, 5, args.seed))
+
+    if not args.no_secondary and not args.no_closure:
+        log("[bench] secondary closure")
+        secondary.append(closure_measure())
 
     cpu = None
     if not args.no_cpu:
