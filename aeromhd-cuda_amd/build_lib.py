@@ -54,10 +54,13 @@ def _run(cmd: list[str]) -> None:
         raise RuntimeError(f"command failed ({r.returncode}): {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
 
 
-def build(out_dir: str | None = None, verbose: bool = False) -> str:
-    """Compile the three TUs and link ``libaeromhd_hip.so``; returns its path."""
+def build(out_dir: str | None = None, verbose: bool = False, extra_flags=()) -> str:
+    """Compile the translation units and link ``libaeromhd_hip.so`` into
+    out_dir (default: this package); extra_flags (e.g. -D tuning macros) go
+    to every compile and get their own object directory."""
     out_dir = out_dir or HERE
-    obj_dir = os.path.join(REPO, "_build", "amhd")
+    tag = "_".join(f.strip("-").replace("=", "") for f in extra_flags)
+    obj_dir = os.path.join(REPO, "_build", "amhd" + ("_" + tag if tag else ""))
     os.makedirs(obj_dir, exist_ok=True)
     cc = hipcc()
     jobs = []
@@ -65,12 +68,13 @@ def build(out_dir: str | None = None, verbose: bool = False) -> str:
     for src, extra in UNITS:
         obj = os.path.join(obj_dir, src.replace(".hip", ".o"))
         objs.append(obj)
-        jobs.append([cc, *COMMON, *extra, "-c", os.path.join(CSRC, src), "-o", obj])
+        jobs.append([cc, *COMMON, *extra, *extra_flags, "-c", os.path.join(CSRC, src), "-o", obj])
     with ThreadPoolExecutor(max_workers=len(jobs)) as ex:
         for cmd, fut in zip(jobs, [ex.submit(_run, c) for c in jobs]):
             if verbose:
                 print(" ".join(cmd), flush=True)
             fut.result()
+    os.makedirs(out_dir, exist_ok=True)
     lib = os.path.join(out_dir, LIB_NAME)
     tmp = lib + ".tmp"
     _run([cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])

@@ -152,6 +152,21 @@ __device__ __forceinline__ real vl_slope(real wm, real w0, real wp) {
     return vl_slope_d<real, P>(w0 - wm, wp - w0);
 }
 
+// Half the van Leer slope: the offset of the face states W +/- slope/2
+// (physics.py:329-336) from the cell value.  The fast path forms
+// ab * rcp(dm + dp + 1e-30) directly, which equals 0.5 * ((2 ab) * rcp(..))
+// bit for bit (scaling by 2 and by 1/2 is exact).
+template <class real, class P>
+__device__ __forceinline__ real vl_half_d(real dm, real dp) {
+    if constexpr (P::exact) {
+        return real(0.5) * vl_slope_d<real, P>(dm, dp);
+    } else {
+        const real ab = dm * dp;
+        const real h = ab * P::rcp(dm + dp + real(1e-30));
+        return ab > real(0) ? h : real(0);
+    }
+}
+
 // ---------------------------------------------------------------- HLLD face
 // Constants of one flux evaluation; derived once per launch so that every
 // per-face quotient the reference evaluates (0.5/ch, ch*ch) has the same
@@ -281,26 +296,58 @@ __device__ __forceinline__ real hlld_face(const real (&wl)[NV], const real (&wr)
     const real byRS = P::divi(byR * (rhoR * dsR * dsR - bx2), dnR, idnR);
     const real bzRS = P::divi(bzR * (rhoR * dsR * dsR - bx2), dnR, idnR);
 
-    const real vbL = vxL * bxf + vyL * byL + vzL * bzL;
-    const real vbLS = sm * bxf + vyLS * byLS + vzLS * bzLS;
-    const real eL = P::divi(pL, k.gm1, k.igm1) + half * rhoL * (vxL * vxL + vyL * vyL + vzL * vzL) + half * bmL2;
-    const real eLS = P::divi((sl - vxL) * eL - ptL * vxL + pts * sm + bxf * (vbL - vbLS), dslsm, idslsm);
-    const real vbR = vxR * bxf + vyR * byR + vzR * bzR;
-    const real vbRS = sm * bxf + vyRS * byRS + vzRS * bzRS;
-    const real eR = P::divi(pR, k.gm1, k.igm1) + half * rhoR * (vxR * vxR + vyR * vyR + vzR * vzR) + half * bmR2;
-    const real eRS = P::divi((sr - vxR) * eR - ptR * vxR + pts * sm + bxf * (vbR - vbRS), dsrsm, idsrsm);
+    // region: supersonic L / R, or the star fan split by the contact.  Only
+    // the side K that holds the face needs its total energy, star energy,
+    // v.B and Alfven speed; the double-star state needs both star states.
+    const bool supL = !GLM && sl >= real(0);
+    const bool supR = !GLM && !supL && sr <= real(0);
+    const bool star = GLM || (!supL && !supR);
+    const bool useL = supL || (star && sm >= real(0));
+
+    const real rho = useL ? rhoL : rhoR;
+    const real vx = useL ? vxL : vxR;
+    const real vy = useL ? vyL : vyR;
+    const real vz = useL ? vzL : vzR;
+    const real by = useL ? byL : byR;
+    const real bz = useL ? bzL : bzR;
+    const real pk = useL ? pL : pR;
+    const real bmk2 = useL ? bmL2 : bmR2;
+    const real pt = useL ? ptL : ptR;
+    const real psi = useL ? psiL : psiR;
+    const real c = useL ? cL : cR;
+    const real sk = useL ? sl : sr;
+    const real dksm = useL ? dslsm : dsrsm;
+    const real idksm = useL ? idslsm : idsrsm;
+    const real rhos = useL ? rhoLS : rhoRS;
+    const real sqk = useL ? sqLS : sqRS;
+    const real rsqk = useL ? rsqLS : rsqRS;
+    const real vys = useL ? vyLS : vyRS;
+    const real vzs = useL ? vzLS : vzRS;
+    const real bys = useL ? byLS : byRS;
+    const real bzs = useL ? bzLS : bzRS;
+
+    // (physics.py:141-154 for side K)
+    const real vb = vx * bxf + vy * by + vz * bz;
+    const real vbs = sm * bxf + vys * bys + vzs * bzs;
+    const real e = P::divi(pk, k.gm1, k.igm1) + half * rho * (vx * vx + vy * vy + vz * vz) + half * bmk2;
+    const real es = P::divi((sk - vx) * e - pt * vx + pts * sm + bxf * (vb - vbs), dksm, idksm);
 
     // double-star state (physics.py:156-172)
     const real sbx = (bxf >= real(0)) ? real(1) : real(-1);
     const real abx = fabs(bxf);
-    // |Bx| / max(sqrt(rho*), 1e-14); rho* >= floor_rho so the fast path
-    // multiplies by 1/sqrt(rho*)
-    real sal = sm - P::divi(abx, fmax(sqLS, real(1e-14)), rsqLS);
-    real sar = sm + P::divi(abx, fmax(sqRS, real(1e-14)), rsqRS);
-    if (sal < sl) sal = sl;
-    if (sal > sm) sal = sm;
-    if (sar < sm) sar = sm;
-    if (sar > sr) sar = sr;
+    // S*_K = SM -/+ |Bx| / max(sqrt(rho*_K), 1e-14), clamped into [SL, SM] / [SM, SR];
+    // rho* >= floor_rho so the fast path multiplies by 1/sqrt(rho*)
+    const real tk = P::divi(abx, fmax(sqk, real(1e-14)), rsqk);
+    real sak;
+    if (useL) {
+        sak = sm - tk;
+        if (sak < sl) sak = sl;
+        if (sak > sm) sak = sm;
+    } else {
+        sak = sm + tk;
+        if (sak < sm) sak = sm;
+        if (sak > sr) sak = sr;
+    }
     const real dds = fmax(sqLS + sqRS, real(1e-14));
     const real idds = P::rcp(dds);
     const real vyDS = P::divi(sqLS * vyLS + sqRS * vyRS + (byRS - byLS) * sbx, dds, idds);
@@ -309,35 +356,9 @@ __device__ __forceinline__ real hlld_face(const real (&wl)[NV], const real (&wr)
     const real bzDS = P::divi(sqLS * bzRS + sqRS * bzLS + sqLS * sqRS * (vzRS - vzLS) * sbx, dds, idds);
     const real vbDS = sm * bxf + vyDS * byDS + vzDS * bzDS;
 
-    // region: supersonic L / R, or the star fan split by the contact
-    const bool supL = !GLM && sl >= real(0);
-    const bool supR = !GLM && !supL && sr <= real(0);
-    const bool star = GLM || (!supL && !supR);
-    const bool useL = supL || (star && sm >= real(0));
-    const bool dbl = star && (useL ? (sal < real(0)) : (sar >= real(0)));
-
-    // side-K quantities
-    const real rho = useL ? rhoL : rhoR;
-    const real vx = useL ? vxL : vxR;
-    const real vy = useL ? vyL : vyR;
-    const real vz = useL ? vzL : vzR;
-    const real by = useL ? byL : byR;
-    const real bz = useL ? bzL : bzR;
-    const real pt = useL ? ptL : ptR;
-    const real e = useL ? eL : eR;
-    const real vb = useL ? vbL : vbR;
-    const real psi = useL ? psiL : psiR;
-    const real c = useL ? cL : cR;
-    const real sk = useL ? sl : sr;
-    const real sak = useL ? sal : sar;
-    const real rhos = useL ? rhoLS : rhoRS;
-    const real vys = useL ? vyLS : vyRS;
-    const real vzs = useL ? vzLS : vzRS;
-    const real bys = useL ? byLS : byRS;
-    const real bzs = useL ? bzLS : bzRS;
-    const real es = useL ? eLS : eRS;
+    const bool dbl = star && (useL ? (sak < real(0)) : (sak >= real(0)));
     // E** (physics.py:171-172): L side subtracts, R side adds
-    const real eds = useL ? (eLS - sqLS * (vbLS - vbDS) * sbx) : (eRS + sqRS * (vbRS - vbDS) * sbx);
+    const real eds = useL ? (es - sqk * (vbs - vbDS) * sbx) : (es + sqk * (vbs - vbDS) * sbx);
 
     // physical flux of side K (physics.py:178-187)
     real fk[NV];

@@ -21,6 +21,14 @@
 #pragma once
 #include "amhd_device.hpp"
 
+// Optional occupancy request for the stage kernels: -DAMHD_STAGE_WAVES_PER_EU=n asks
+// the compiler to fit n waves per SIMD (register budget 512/n per lane).
+#if defined(AMHD_STAGE_WAVES_PER_EU) && AMHD_STAGE_WAVES_PER_EU > 0
+#define AMHD_STAGE_OCC __attribute__((amdgpu_waves_per_eu(AMHD_STAGE_WAVES_PER_EU)))
+#else
+#define AMHD_STAGE_OCC
+#endif
+
 namespace amhd {
 
 constexpr int WAVE = 64;
@@ -102,22 +110,32 @@ __device__ __forceinline__ void load_cell(const real* __restrict__ base, long pl
 // m[0] collects cf^2 = gamma p/rho + B^2/rho (sqrt is monotone, so
 // sqrt(max cf^2) == max cf; the square root is taken once per step),
 // m[1] |vx|+|vy|, m[2] |vx|, m[3] |vy|.  FP64 state: the reference's
-// expression; FP32 state: (gamma p + B^2)/rho in FP64.
+// expression; FP32 state: (gamma p + B^2)/rho in FP32, maxima in FP64.
 template <class real>
 __device__ __forceinline__ void dt_terms(const real (&u)[NV], double gamma, double (&m)[NPART_DT]) {
-    double ud[NV], w[NV];
-#pragma unroll
-    for (int v = 0; v < NV; ++v) ud[v] = double(u[v]);
-    cons_to_prim<double, ExactMath<double>>(ud, w, gamma, false);
-    const double b2 = w[iBX] * w[iBX] + w[iBY] * w[iBY] + w[iBZ] * w[iBZ];
-    double cf2;
-    if constexpr (sizeof(real) == sizeof(double)) cf2 = gamma * w[iPR] / w[iRHO] + b2 / w[iRHO];
-    else cf2 = (gamma * w[iPR] + b2) / w[iRHO];
-    const double ax = fabs(w[iVX]), ay = fabs(w[iVY]);
-    m[0] = fmax(m[0], cf2);
-    m[1] = fmax(m[1], ax + ay);
-    m[2] = fmax(m[2], ax);
-    m[3] = fmax(m[3], ay);
+    if constexpr (sizeof(real) == sizeof(double)) {
+        // FP64 state: the reference's expressions (solver.py:208-214)
+        double w[NV];
+        cons_to_prim<double, ExactMath<double>>(u, w, gamma, false);
+        const double b2 = w[iBX] * w[iBX] + w[iBY] * w[iBY] + w[iBZ] * w[iBZ];
+        const double cf2 = gamma * w[iPR] / w[iRHO] + b2 / w[iRHO];
+        const double ax = fabs(w[iVX]), ay = fabs(w[iVY]);
+        m[0] = fmax(m[0], cf2);
+        m[1] = fmax(m[1], ax + ay);
+        m[2] = fmax(m[2], ax);
+        m[3] = fmax(m[3], ay);
+    } else {
+        // FP32 state: FP32 terms (cf^2 = (gamma p + B^2) / rho), FP64 maxima
+        float w[NV];
+        cons_to_prim<float, FastMath<float>>(u, w, float(gamma), false);
+        const float b2 = w[iBX] * w[iBX] + w[iBY] * w[iBY] + w[iBZ] * w[iBZ];
+        const float cf2 = (float(gamma) * w[iPR] + b2) * __builtin_amdgcn_rcpf(w[iRHO]);
+        const float ax = fabsf(w[iVX]), ay = fabsf(w[iVY]);
+        m[0] = fmax(m[0], double(cf2));
+        m[1] = fmax(m[1], double(ax + ay));
+        m[2] = fmax(m[2], double(ax));
+        m[3] = fmax(m[3], double(ay));
+    }
 }
 
 // Boundary energy-flux density of one cell (solver.py:233-239), FP64.
@@ -206,11 +224,11 @@ __device__ __forceinline__ void stage_wave(const StageParams<real>& a, const int
         for (int v = 0; v < NV; ++v) {
             const real dmm = wm1[v] - wm2[v], dm0 = A[v] - wm1[v];
             dxp[v] = B[v] - A[v];
-            const real s_m1 = vl_slope_d<real, P>(dmm, dm0);
-            const real s_0 = vl_slope_d<real, P>(dm0, dxp[v]);
-            fa[v] = wm1[v] + real(0.5) * s_m1;
-            fb[v] = A[v] - real(0.5) * s_0;
-            xl[v] = A[v] + real(0.5) * s_0;
+            const real h_m1 = vl_half_d<real, P>(dmm, dm0);
+            const real h_0 = vl_half_d<real, P>(dm0, dxp[v]);
+            fa[v] = wm1[v] + h_m1;
+            fb[v] = A[v] - h_0;
+            xl[v] = A[v] + h_0;
         }
         floor_face(fa); floor_face(fb); floor_face(xl);
         hlld_face<real, P, GLM>(fa, fb, F0, kc);
@@ -241,10 +259,10 @@ __device__ __forceinline__ void stage_wave(const StageParams<real>& a, const int
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
                 const real dn = wp2[v] - wp1[v];
-                const real d1 = vl_slope_d<real, P>(dxp[v], dn);
+                const real h1 = vl_half_d<real, P>(dxp[v], dn);
                 dxp[v] = dn;
-                xr[v] = wp1[v] - real(0.5) * d1;
-                xn[v] = wp1[v] + real(0.5) * d1;
+                xr[v] = wp1[v] - h1;
+                xn[v] = wp1[v] + h1;
             }
             floor_face(xr);
             hlld_face<real, P, GLM>(xl, xr, fxr, kc);
@@ -264,9 +282,9 @@ __device__ __forceinline__ void stage_wave(const StageParams<real>& a, const int
                 const real dp = wnv - w0[v];
                 const real dm = from_prev(dp);
                 if (v == iBY) { by_next = wnv; by_prev = from_prev(w0[v]); }
-                const real d = vl_slope_d<real, P>(dm, dp);
-                yl[v] = w0[v] + real(0.5) * d;
-                yr[v] = from_next(w0[v] - real(0.5) * d);
+                const real h = vl_half_d<real, P>(dm, dp);
+                yl[v] = w0[v] + h;
+                yr[v] = from_next(w0[v] - h);
             }
             floor_face(yl); floor_face(yr);
             hlld_face_y<real, P, GLM>(yl, yr, fy, kc);
@@ -370,7 +388,7 @@ __device__ __forceinline__ void stage_wave(const StageParams<real>& a, const int
 }
 
 template <class real, class P, int STAGE, bool POWELL, bool GLM>
-__global__ __launch_bounds__(WAVE* WPB) void stage_kernel(StageParams<real> a) {
+__global__ __launch_bounds__(WAVE* WPB) AMHD_STAGE_OCC void stage_kernel(StageParams<real> a) {
     const int lane = threadIdx.x & (WAVE - 1);
     // wave-uniform quantities live in scalar registers
     const int wave = __builtin_amdgcn_readfirstlane(xcd_logical_block() * WPB + (int)(threadIdx.x >> 6));
@@ -763,7 +781,7 @@ struct BatchInst {
 };
 
 template <class real, class P, int STAGE>
-__global__ __launch_bounds__(WAVE* WPB) void batch_stage_kernel(const BatchInst* __restrict__ insts, int n_inst,
+__global__ __launch_bounds__(WAVE* WPB) AMHD_STAGE_OCC void batch_stage_kernel(const BatchInst* __restrict__ insts, int n_inst,
                                                                 int n_tasks) {
     const int lane = threadIdx.x & (WAVE - 1);
     const int first = __builtin_amdgcn_readfirstlane(xcd_logical_block() * WPB + (int)(threadIdx.x >> 6));
