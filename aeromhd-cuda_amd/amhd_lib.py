@@ -50,6 +50,13 @@ SIGNATURES = {
     "amhd_reset_boundary_energy": (_i, [_vp, _d]),
     "amhd_compute_rhs": (_i, [_vp, _vp, _vp, _d]),
     "amhd_global_sums": (_i, [_vp, _dp]),
+    "amhd_concentration_profile": (_i, [_vp, _dp, _dp]),
+    "amhd_set_halo": (_i, [_vp, _vp, _vp]),
+    "amhd_stage": (_i, [_vp, _i, _d, _d, _d]),
+    "amhd_edge_rows": (_i, [_vp, _i, _i, _vp]),
+    "amhd_cfl_maxima": (_i, [_vp, _dp]),
+    "amhd_dt_from_maxima": (_i, [_dp, _d, _d, _d, _dp, _dp]),
+    "amhd_synchronize": (_i, [_vp]),
     "amhd_rewind": (_i, [_vp]),
     "amhd_state_check": (_i, [_vp, _dp]),
     "amhd_state_ptr": (_i, [_vp, C.POINTER(_vp)]),

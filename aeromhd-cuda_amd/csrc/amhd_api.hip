@@ -93,6 +93,7 @@ struct Ctx {
     StepCtl* ctl = nullptr;
     hipStream_t stream = nullptr;
     hipGraphExec_t graph[3] = {nullptr, nullptr, nullptr};
+    const void* halo[2] = {nullptr, nullptr};   // x-slab halo buffers (lo, hi), caller-owned
 };
 
 size_t esize(int dtype) { return dtype == AMHD_F32 ? sizeof(float) : sizeof(double); }
@@ -123,6 +124,8 @@ StageParams<real> stage_args(const Ctx* c, int stage, const real* u0, const real
     a.ctl = nullptr;
     a.part_dt = stage == 3 ? c->part_dt : nullptr;
     a.part_bf = stage == 1 ? c->part_bf : nullptr;
+    a.halo_lo = (const real*)c->halo[0];
+    a.halo_hi = (const real*)c->halo[1];
     return a;
 }
 
@@ -218,6 +221,45 @@ __global__ __launch_bounds__(256) void global_sums_kernel(const real* __restrict
 }
 
 constexpr int GSUM_BLOCKS = 512;
+
+// y-profile of the concentration (solver.py:340, 352-364): for interior
+// row i, C = clip(rhoC * (1 / max(rho, 1e-12)), 0, 1) (cons_to_prim's C),
+// out[i] = mean_j C and out[nx + i] = mean_j (C - mean)^2, both in FP64 with
+// a fixed summation order.  One workgroup per row.
+template <class real>
+__global__ __launch_bounds__(256) void conc_profile_kernel(const real* __restrict__ u, long plane, int pitch, int nx,
+                                                           int ny, double* __restrict__ out) {
+    const int i = blockIdx.x;
+    const real* r = u + RHO * plane + (long)i * pitch;
+    const real* rc = u + RHOC * plane + (long)i * pitch;
+    auto conc = [&](int j) {
+        const double rho = fmax(double(r[j]), 1e-12);
+        const double c = double(rc[j]) * (1.0 / rho);
+        return c < 0.0 ? 0.0 : (c > 1.0 ? 1.0 : c);
+    };
+    __shared__ double sh[4];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    auto block_sum = [&](double v) {
+        v = wave_sum(v);
+        __syncthreads();
+        if (lane == 0) sh[w] = v;
+        __syncthreads();
+        return (sh[0] + sh[1]) + (sh[2] + sh[3]);
+    };
+    double s = 0.0;
+    for (int j = threadIdx.x; j < ny; j += blockDim.x) s += conc(j);
+    const double mean = block_sum(s) / ny;
+    double s2 = 0.0;
+    for (int j = threadIdx.x; j < ny; j += blockDim.x) {
+        const double d = conc(j) - mean;
+        s2 += d * d;
+    }
+    const double var = block_sum(s2) / ny;
+    if (threadIdx.x == 0) {
+        out[i] = mean;
+        out[nx + i] = var;
+    }
+}
 
 }  // namespace
 
@@ -366,6 +408,80 @@ int amhd_step_ssprk3(void* ctx, double dt, double ch, double psi_decay) {
     c->cur = (c->cur + 1) % 3;
     c->n_part = n_waves(c);
     c->part_valid = true;
+    return 0;
+}
+
+int amhd_set_halo(void* ctx, const void* lo, const void* hi) {
+    if (!ctx) return fail(-1, "amhd_set_halo: null context");
+    Ctx* c = (Ctx*)ctx;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (c->halo[0] != lo || c->halo[1] != hi) drop_graphs(c);
+    c->halo[0] = lo;
+    c->halo[1] = hi;
+    return 0;
+}
+
+int amhd_stage(void* ctx, int stage, double dt, double ch, double psi_decay) {
+    if (!ctx) return fail(-1, "amhd_stage: null context");
+    if (stage < 1 || stage > 3) return fail(-1, "amhd_stage: stage must be 1, 2 or 3 (got %d)", stage);
+    Ctx* c = (Ctx*)ctx;
+    void* u0 = c->buf[c->cur];
+    void* u1 = c->buf[(c->cur + 1) % 3];
+    void* u2 = c->buf[(c->cur + 2) % 3];
+    if (stage == 1) HIP_TRY(launch_stage_any(c, 1, u0, u0, u1, dt, ch, psi_decay, nullptr));
+    else if (stage == 2) HIP_TRY(launch_stage_any(c, 2, u0, u1, u2, dt, ch, psi_decay, nullptr));
+    else {
+        HIP_TRY(launch_stage_any(c, 3, u0, u2, u1, dt, ch, psi_decay, nullptr));
+        HIP_TRY(enqueue_ctl(c, 0, dt, 0.0, CTL_HOST_STEP_END, c->ctl));
+        c->cur = (c->cur + 1) % 3;
+        c->n_part = n_waves(c);
+        c->part_valid = true;
+    }
+    return 0;
+}
+
+int amhd_edge_rows(void* ctx, int stage, int side, void* dst) {
+    if (!ctx || !dst) return fail(-1, "amhd_edge_rows: null argument");
+    if (stage < 1 || stage > 3 || (side != 0 && side != 1)) return fail(-1, "amhd_edge_rows: bad stage or side");
+    Ctx* c = (Ctx*)ctx;
+    if (c->nx < 2) return fail(-1, "amhd_edge_rows: the slab needs at least 2 rows");
+    const void* src = c->buf[(c->cur + stage - 1) % 3];   // the stage's input
+    const size_t es = esize(c->dtype);
+    const size_t row0 = side == 0 ? 0 : (size_t)(c->nx - 2);
+    HIP_TRY(hipMemcpy2DAsync(dst, 2 * (size_t)c->pitch * es, (const char*)src + row0 * c->pitch * es,
+                             (size_t)c->plane * es, 2 * (size_t)c->pitch * es, NV, hipMemcpyDeviceToDevice, c->stream));
+    return 0;
+}
+
+int amhd_cfl_maxima(void* ctx, double m[4]) {
+    if (!ctx || !m) return fail(-1, "amhd_cfl_maxima: null argument");
+    Ctx* c = (Ctx*)ctx;
+    int rc = ensure_partials(c);
+    if (rc) return rc;
+    std::vector<double> h((size_t)c->n_part * NPART_DT);
+    HIP_TRY(hipMemcpyAsync(h.data(), c->part_dt, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    for (int q = 0; q < NPART_DT; ++q) m[q] = 0.0;
+    for (int k = 0; k < c->n_part; ++k)
+        for (int q = 0; q < NPART_DT; ++q) m[q] = std::max(m[q], h[(size_t)k * NPART_DT + q]);
+    return 0;
+}
+
+int amhd_dt_from_maxima(const double m[4], double cfl, double dx, double dy, double* dt, double* ch) {
+    if (!m || !dt || !ch || !(dx > 0) || !(dy > 0)) return fail(-1, "amhd_dt_from_maxima: bad argument");
+    // the expressions of the device step control (solver.py:211-223)
+    const double cfmax = std::sqrt(m[0]);
+    const double c_h = std::fmax(cfmax + m[1], 1.0) * 1.5;
+    const double sx = m[2] + c_h, sy = m[3] + c_h;
+    const double sm = std::fmax(sx / dx, sy / dy);
+    *dt = (sm < 1e-14) ? cfl * std::fmin(dx, dy) : cfl / sm;
+    *ch = c_h;
+    return 0;
+}
+
+int amhd_synchronize(void* ctx) {
+    if (!ctx) return fail(-1, "amhd_synchronize: null context");
+    HIP_TRY(hipStreamSynchronize(((Ctx*)ctx)->stream));
     return 0;
 }
 
@@ -580,6 +696,23 @@ int amhd_state_ptr(void* ctx, void** state) {
     HIP_TRY(hipStreamSynchronize(c->stream));
     *state = c->buf[c->cur];
     c->part_valid = false;   // the caller may edit the state: recompute the CFL partials
+    return 0;
+}
+
+int amhd_concentration_profile(void* ctx, double* mean, double* var) {
+    if (!ctx || !mean || !var) return fail(-1, "amhd_concentration_profile: null argument");
+    Ctx* c = (Ctx*)ctx;
+    double* d = c->staging;   // 2 nx doubles of scratch (the staging buffer holds far more)
+    if (c->dtype == AMHD_F64)
+        conc_profile_kernel<double><<<c->nx, 256, 0, c->stream>>>((const double*)c->buf[c->cur], c->plane, c->pitch,
+                                                                 c->nx, c->ny, d);
+    else
+        conc_profile_kernel<float><<<c->nx, 256, 0, c->stream>>>((const float*)c->buf[c->cur], c->plane, c->pitch,
+                                                                c->nx, c->ny, d);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(mean, d, sizeof(double) * c->nx, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(var, d + c->nx, sizeof(double) * c->nx, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
     return 0;
 }
 

@@ -62,6 +62,11 @@ template <class real> struct StageParams {
     const StepCtl* ctl;     // device-resident dt/ch/decay
     double* part_dt;        // STAGE 3: [n_waves][4]
     double* part_bf;        // STAGE 1: [n_strips][2] (non-periodic x only)
+    // x-slab halos: rows -2, -1 (halo_lo) and nx, nx+1 (halo_hi) of the stage
+    // input taken from neighbouring slabs, 10 planes of 2 rows of `pitch`
+    // cells; null = the domain boundary (wrap / clamp, psi = 0).
+    const real* halo_lo;
+    const real* halo_hi;
 };
 
 // ---------------------------------------------------------- buffer access
@@ -189,7 +194,12 @@ __device__ __forceinline__ void stage_wave(const StageParams<real>& a, const int
     const __amdgpu_buffer_rsrc_t r_uo = make_rsrc(a.uo, a.out_bytes);
     const bool xper = a.bcx_periodic != 0;
 
-    // x ghost rows: wrap (periodic) or clamp; psi = 0 on non-periodic x ghosts
+    // x ghost rows: a neighbouring slab's halo rows, else wrap (periodic) or
+    // clamp with psi = 0 on non-periodic x ghosts
+    const bool hlo = a.halo_lo != nullptr, hhi = a.halo_hi != nullptr;
+    const unsigned hbytes = 2u * rb * NV;
+    const __amdgpu_buffer_rsrc_t r_hlo = make_rsrc(hlo ? (const void*)a.halo_lo : a.us, hlo ? hbytes : 0u);
+    const __amdgpu_buffer_rsrc_t r_hhi = make_rsrc(hhi ? (const void*)a.halo_hi : a.us, hhi ? hbytes : 0u);
     auto row_src = [&](int i) -> int {
         if (i < 0) return xper ? i + nx : 0;
         if (i >= nx) return xper ? i - nx : nx - 1;
@@ -200,10 +210,22 @@ __device__ __forceinline__ void stage_wave(const StageParams<real>& a, const int
 #pragma unroll
         for (int v = 0; v < NV; ++v) u[v] = bload(real(0), r, vo, so + (unsigned)v * pb);
     };
+    // stage-input row i for any i in [-2, nx + 2) (wave-uniform branch)
+    auto load_in = [&](int i, real (&u)[NV]) {
+        if ((i < 0 && hlo) || (i >= nx && hhi)) {
+            const __amdgpu_buffer_rsrc_t r = i < 0 ? r_hlo : r_hhi;
+            const unsigned so = (unsigned)(i < 0 ? i + 2 : i - nx) * rb;
+#pragma unroll
+            for (int v = 0; v < NV; ++v) u[v] = bload(real(0), r, vo, so + (unsigned)v * 2u * rb);
+        } else {
+            load_row(r_us, row_src(i), u);
+        }
+    };
+    auto psi_ghost = [&](int i) { return !xper && ((i < 0 && !hlo) || (i >= nx && !hhi)); };
     auto prim_row = [&](int i, real (&w)[NV]) {
         real u[NV];
-        load_row(r_us, row_src(i), u);
-        cons_to_prim<real, P>(u, w, kc.gamma, !xper && (i < 0 || i >= nx));
+        load_in(i, u);
+        cons_to_prim<real, P>(u, w, kc.gamma, psi_ghost(i));
     };
 
     // prologue: W at rows i0, i0+1, the x-differences and the flux through
@@ -238,14 +260,14 @@ __device__ __forceinline__ void stage_wave(const StageParams<real>& a, const int
     double mx[NPART_DT] = {0.0, 0.0, 0.0, 0.0};
     double bf_sum[2] = {0.0, 0.0};
     real upre[NV];
-    load_row(r_us, row_src(i0 + 2), upre);
+    load_in(i0 + 2, upre);
 
     // One output row i.  w0 = W_i, wp1 = W_{i+1}; wp2 receives W_{i+2};
     // fxl = F_{i-1/2}; fxr receives F_{i+1/2}.
     auto row = [&](int i, real (&w0)[NV], real (&wp1)[NV], real (&wp2)[NV], const real (&fxl)[NV],
                    real (&fxr)[NV]) __attribute__((always_inline)) {
-        cons_to_prim<real, P>(upre, wp2, kc.gamma, !xper && (i + 2 >= nx));
-        if (i + 1 < i1) load_row(r_us, row_src(i + 3), upre);
+        cons_to_prim<real, P>(upre, wp2, kc.gamma, psi_ghost(i + 2));
+        if (i + 1 < i1) load_in(i + 3, upre);
 
         // stage input and base at this row (halo lanes read a valid
         // remapped column and never store)
@@ -346,9 +368,9 @@ __device__ __forceinline__ void stage_wave(const StageParams<real>& a, const int
             }
             if constexpr (STAGE == 3) dt_terms(unew, a.gamma, mx);
             if constexpr (STAGE == 1) {
-                if (!xper) {
-                    if (i == 0) bf_sum[0] += bflux_term(ucur, a.gamma, a.dy);
-                    if (i == nx - 1) bf_sum[1] += bflux_term(ucur, a.gamma, a.dy);
+                if (!xper) {   // physical x boundaries only (not slab edges)
+                    if (i == 0 && !hlo) bf_sum[0] += bflux_term(ucur, a.gamma, a.dy);
+                    if (i == nx - 1 && !hhi) bf_sum[1] += bflux_term(ucur, a.gamma, a.dy);
                 }
             }
         }
@@ -811,6 +833,8 @@ __global__ __launch_bounds__(WAVE* WPB) AMHD_STAGE_OCC void batch_stage_kernel(c
         a.ctl = d.ctl;
         a.part_dt = STAGE == 3 ? d.part_dt : nullptr;
         a.part_bf = STAGE == 1 ? d.part_bf : nullptr;
+        a.halo_lo = nullptr;
+        a.halo_hi = nullptr;
         const int wave = task - d.task_base;
         if (d.powell) stage_wave<real, P, STAGE, true, true>(a, wave, lane);
         else stage_wave<real, P, STAGE, false, true>(a, wave, lane);

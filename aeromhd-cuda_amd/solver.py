@@ -148,6 +148,14 @@ class Context:
         _L.check(self.lib.amhd_global_sums(self.h, out), "amhd_global_sums")
         return tuple(out)
 
+    def concentration_profile(self):
+        mean = np.empty(self.nx, dtype=np.float64)
+        var = np.empty(self.nx, dtype=np.float64)
+        _L.check(self.lib.amhd_concentration_profile(self.h, mean.ctypes.data_as(C.POINTER(C.c_double)),
+                                                     var.ctypes.data_as(C.POINTER(C.c_double))),
+                 "amhd_concentration_profile")
+        return mean, var
+
 
 class MHDSolver:
     """2-D ideal MHD: MUSCL/van Leer + HLLD, SSP-RK3, GLM + Powell
@@ -183,6 +191,7 @@ class MHDSolver:
         self.diag_energy_total = []
         self.diag_boundary_flux_cumulative = []
         self.snapshots = {}
+        self._e0 = None          # energy of the first device_diagnostics call (drift reference)
         self._rho_light = 1.0
         self._rho_heavy = 1.0
         self._rh_results = None
@@ -231,6 +240,7 @@ class MHDSolver:
         self.t = 0.0
         self.step = 0
         self._cumulative_boundary_energy = 0.0
+        self._e0 = None
         self._rho_light = 1.0
         self._rho_heavy = rho_h
 
@@ -303,6 +313,37 @@ class MHDSolver:
         divB as solver.py:419-428."""
         self._sync_state()
         return self.ctx.global_sums()
+
+    def device_diagnostics(self) -> dict:
+        """The diagnostics the time loop needs, reduced on the device in
+        FP64 without downloading the state: total mass and energy
+        (solver.py:430), max / L2 divB (solver.py:419-428), the integral and
+        threshold mixing widths and the mixedness (solver.py:352-371, from
+        the device concentration profile; the O(nx) trapezoid sums on the
+        host), and the energy drift raw and corrected by the boundary flux
+        (solver.py:507-511) against the first call's energy."""
+        mass, energy, div_max, div_l2 = self.global_sums()
+        cbar, cvar = self.ctx.concentration_profile()
+        x = self.cfg.x_min + (np.arange(self.cfg.nx) + 0.5) * self.cfg.dx
+        trap = lambda y, xx: float(np.sum(0.5 * (y[1:] + y[:-1]) * np.diff(xx))) if len(xx) > 1 else 0.0
+        mw_integral = trap(cbar * (1.0 - cbar), x)
+        mixed = np.where((cbar > 0.01) & (cbar < 0.99))[0]
+        mw_thresh = float(x[mixed[-1]] - x[mixed[0]]) if len(mixed) > 1 else 0.0
+        if len(mixed) > 1:
+            num = trap(cvar[mixed], x[mixed])
+            den = trap(np.maximum(cbar * (1.0 - cbar), 1e-14)[mixed], x[mixed])
+            mixedness = float(np.clip(1.0 - num / max(den, 1e-14), 0.0, 1.0))
+        else:
+            mixedness = 0.0
+        if self._e0 is None:
+            self._e0 = energy
+        bf = self._cumulative_boundary_energy
+        e0 = self._e0
+        return {"mass_total": mass, "energy_total": energy, "divB_max": div_max, "divB_L2": div_l2,
+                "mw_integral": mw_integral, "mw_thresh": mw_thresh, "mixedness": mixedness,
+                "energy_drift_raw_pct": abs(energy - e0) / max(abs(e0), 1e-14) * 100,
+                "energy_drift_corrected_pct": abs(energy - bf - e0) / max(abs(e0), 1e-14) * 100,
+                "boundary_flux": bf}
 
     def compute_diagnostics(self):
         """Diagnostics of solver.py:335-444 on the downloaded state."""

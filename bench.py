@@ -12,8 +12,9 @@ kernel (algorithmic bytes 8 * 10 * sizeof(real) per cell-update divided by
 the stage kernels' device time, HIP events on the context stream over the
 timed region), and the CPU oracle timed on this host on a bounded sample.
 Secondary records: the 1024x512 RMI in FP64 and FP32, the batched form (64
-instances), and the learned closure's fine-equivalent rate over the FP64
-fine solve with its mixing-width and density errors.
+instances), the x-slab decomposition (4 slabs with halo exchange, one
+GPU), and the learned closure's fine-equivalent rate over the FP64 fine
+solve with its mixing-width and density errors.
 
   python bench.py [--steps K] [--warmup W] [--workload sieve2048|rmi1024-fp64|rmi1024-fp32]
 """
@@ -147,6 +148,33 @@ def batch_measure(n: int, steps: int, warmup: int, seed: int, seq_steps: int = 2
             "sequential_value": round(cells * seq_steps / seq_wall / 1e6, 2)}
 
 
+def slab_measure(n_slabs: int = 4, steps: int = 40, warmup: int = 3, seed: int = 0):
+    """The x-slab decomposition of the 2048x1024 FP32 sieve into n_slabs
+    slabs (512x1024 each for 4: one GPU's share on 4 GPUs), all run on this
+    GPU with a halo exchange before every RK stage and the dt taken over all
+    slabs; host-driven (no graph), so it also carries the per-stage host
+    synchronisations a multi-GPU step would have."""
+    import torch
+    from amhd_ics import cons_from_prim_host, magnetic_sieve
+    from amhd_slab import SlabSolver
+
+    cfg, W = magnetic_sieve(2048, 1024, precision="fp32", seed=seed, max_steps=10 ** 6)
+    cfg.t_end = 1e30
+    sl = SlabSolver(cfg, n_slabs)
+    sl.upload(cons_from_prim_host(W, cfg.gamma))
+    sl.advance(warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    n = sl.advance(steps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    i0, i1 = sl.bounds[0]
+    return {"workload": f"x-slab decomposition of the 2048x1024 FP32 sieve: {n_slabs} slabs of {i1 - i0}x1024 "
+                        "on one GPU, halo exchange before each RK stage, global dt (host-driven)",
+            "value": round(cfg.nx * cfg.ny * n / wall / 1e6, 2), "unit": "Mcell-updates/s",
+            "ms_per_step": round(wall / n * 1e3, 4), "steps": n}
+
+
 def closure_measure(seed: int = 210, fine_steps: int = 2000):
     """The learned closure on one held-out magnetic-sieve seed: coarse
     256x128 FP32 with the shipped weights vs the plain coarse solver, both
@@ -170,27 +198,64 @@ def closure_measure(seed: int = 210, fine_steps: int = 2000):
             "wall_closure_s": round(r["wall_closure_s"], 4)}
 
 
-def cpu_baseline(seed: int, steps: int, threads: int):
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(seed: int, steps: int, threads: int, steps_1core: int = 2):
     """The CPU oracle (C restatement of the reference math, FP64, OpenMP
     over x rows) on the 2048 x 1024 sieve: `steps` SSP-RK3 steps with the
-    reference's per-step dt."""
+    reference's per-step dt on `threads` threads, and `steps_1core` on one."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle
     from amhd_ics import cons_from_prim_host, magnetic_sieve
 
     oracle.build()
-    oracle.set_threads(threads)
     cfg, W = magnetic_sieve(2048, 1024, precision="fp64", seed=seed)
     cfg.t_end = 1e30
-    U = cons_from_prim_host(W, cfg.gamma)
-    U, *_ = oracle.run_loop(U, cfg, 1)          # untimed first step (page-in)
+    U0 = cons_from_prim_host(W, cfg.gamma)
+    oracle.set_threads(threads)
+    U, *_ = oracle.run_loop(U0, cfg, 1)          # untimed first step (page-in)
     t0 = time.perf_counter()
-    U, _, n, _, _ = oracle.run_loop(U, cfg, steps)
+    _, _, n, _, _ = oracle.run_loop(U, cfg, steps)
     wall = time.perf_counter() - t0
-    return {"value": round(cfg.nx * cfg.ny * n / wall / 1e6, 3), "unit": "Mcell-updates/s", "cores": threads,
+    oracle.set_threads(1)
+    t1 = time.perf_counter()
+    _, _, n1, _, _ = oracle.run_loop(U, cfg, steps_1core)
+    wall1 = time.perf_counter() - t1
+    cells = cfg.nx * cfg.ny
+    return {"value": round(cells * n / wall / 1e6, 3), "unit": "Mcell-updates/s", "cores": threads,
             "kind": "port",
             "sample": f"{n} SSP-RK3 steps (with per-step CFL dt) of the 2048x1024 magnetic-sieve RMI in FP64 "
-                      f"(oracle/oracle.c, gcc -O3 -fopenmp, {threads} threads), {wall:.1f} s"}
+                      f"(oracle/oracle.c, gcc -O3 -fopenmp, {threads} threads), {wall:.1f} s; "
+                      f"{n1} steps on 1 thread, {wall1:.1f} s",
+            "single_core_value": round(cells * n1 / wall1 / 1e6, 3),
+            "nproc": os.cpu_count(), "cpu_model": cpu_model()}
+
+
+def copy_bandwidth(mib: int = 1024, reps: int = 10) -> float:
+    """GB/s of a plain 1 GiB device-to-device copy (read + write bytes over
+    the event-timed duration), the practical ceiling the roofline is also
+    quoted against."""
+    import torch
+
+    x = torch.empty(mib * (1 << 20) // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    y = torch.empty_like(x)
+    y.copy_(x)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        y.copy_(x)
+    e1.record()
+    torch.cuda.synchronize()
+    return 2.0 * mib * (1 << 20) * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9
 
 
 def copy_calibration(mib: int = 1024, reps: int = 5):
@@ -225,7 +290,8 @@ def main():
     ap.add_argument("--workload", default="sieve2048")
     ap.add_argument("--rows", type=int, default=0)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--cpu-steps", type=int, default=6)
+    ap.add_argument("--cpu-steps", type=int, default=16)
+    ap.add_argument("--cpu-steps-1core", type=int, default=2)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-secondary", action="store_true")
@@ -263,6 +329,10 @@ def main():
         log("[bench] secondary batch64")
         secondary.append(batch_measure(64, args.batch_steps, 5, args.seed))
 
+    if not args.no_secondary:
+        log("[bench] secondary slabs")
+        secondary.append(slab_measure())
+
     if not args.no_secondary and not args.no_closure:
         log("[bench] secondary closure")
         secondary.append(closure_measure())
@@ -271,8 +341,10 @@ def main():
     if not args.no_cpu:
         threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
         log(f"[bench] cpu baseline: oracle, {threads} threads, {args.cpu_steps} steps")
-        cpu = cpu_baseline(args.seed, args.cpu_steps, threads)
+        cpu = cpu_baseline(args.seed, args.cpu_steps, threads, args.cpu_steps_1core)
 
+    copy_gbs = copy_bandwidth()
+    log(f"[bench] device copy {copy_gbs:.0f} GB/s")
     traffic = traffic_from_profile(args.workload)
     avg_launch_bytes = sum(m["bytes_per_launch"]) / 3
     out = {
@@ -297,7 +369,9 @@ def main():
                      "algorithmic_bytes_per_launch": [int(b) for b in m["bytes_per_launch"]],
                      "avg_launch_bytes": int(avg_launch_bytes),
                      "stage_avg_us": [round(u, 2) for u in m["stage_avg_us"]],
-                     "ctl_avg_us": round(m["ctl_avg_us"], 2)},
+                     "ctl_avg_us": round(m["ctl_avg_us"], 2),
+                     "copy_GBps": round(copy_gbs, 1),
+                     "frac_of_copy": round(m["achieved_gbs"] / copy_gbs, 4)},
         "cpu_baseline": cpu,
         "profiled_pass_value": round(m["profiled_mcups"], 2),
         "secondary": secondary,

@@ -76,6 +76,31 @@ int amhd_compute_dt(void* ctx, double cfl, double* dt, double* ch);
  * (solver.py:331-333).  Asynchronous on the context stream. */
 int amhd_step_ssprk3(void* ctx, double dt, double ch, double psi_decay);
 
+/* ---- x-slab decomposition (one slab per context; the per-GPU part of the
+ * multi-GPU layout, no counterpart in the reference).  A slab is a context
+ * over rows [i0, i1) of the global grid (x_min and dx of the global grid).
+ * amhd_set_halo: caller-owned halo buffers for the rows just outside the
+ * slab, each 10 planes of 2 rows of ny cells in the context's precision
+ * (plane stride 2*ny): lo = rows i0-2, i0-1, hi = rows i1, i1+1 of the stage
+ * input; NULL = that side is the domain boundary (the params' BC).
+ * amhd_stage: one SSP-RK3 stage (1, 2, 3) of amhd_step_ssprk3 on its own, so
+ * that halos can be refreshed between stages; stage 3 completes the step.
+ * amhd_edge_rows: copy the slab's first (side 0) or last (side 1) two rows
+ * of the given stage's input into a neighbour's halo buffer, on the
+ * context's stream.
+ * amhd_cfl_maxima: the four CFL maxima of the current state, {max cf^2,
+ * max(|vx|+|vy|), max|vx|, max|vy|} (synchronises); the maxima of the whole
+ * grid are their maxima over the slabs.
+ * amhd_dt_from_maxima: dt and ch of solver.py:206-223 from such maxima (the
+ * device step control's expressions, on the host).
+ * amhd_synchronize: wait for the context's stream. */
+int amhd_set_halo(void* ctx, const void* lo, const void* hi);
+int amhd_stage(void* ctx, int stage, double dt, double ch, double psi_decay);
+int amhd_edge_rows(void* ctx, int stage, int side, void* dst);
+int amhd_cfl_maxima(void* ctx, double m[4]);
+int amhd_dt_from_maxima(const double m[4], double cfl, double dx, double dy, double* dt, double* ch);
+int amhd_synchronize(void* ctx);
+
 /* Device-resident time loop of MHDSolver.run (solver.py:475-482) without
  * diagnostics: up to n_steps steps, stopping at t_end or max_steps (counted
  * in *step_inout) like the reference loop.  dt, ch and the psi decay are
@@ -105,6 +130,13 @@ int amhd_compute_rhs(void* ctx, double* U_dev, double* R_dev, double ch);
  * out[1] = sum(E)*dx*dy (solver.py:430), out[2] = max|divB|,
  * out[3] = sqrt(mean(divB^2)) with divB as solver.py:419-428. */
 int amhd_global_sums(void* ctx, double out[4]);
+
+/* The concentration profile behind the mixing diagnostics
+ * (solver.py:340, 352-364): for each interior x row i, with
+ * C = clip(rhoC * (1/max(rho, 1e-12)), 0, 1) as cons_to_prim gives it,
+ * mean[i] = mean over y of C and var[i] = mean over y of (C - mean[i])^2,
+ * FP64 (host arrays of nx; synchronises). */
+int amhd_concentration_profile(void* ctx, double* mean, double* var);
 
 /* Learned-closure hooks (no counterpart in the reference).
  * amhd_rewind: after a host-driven step, make the previous state current

@@ -251,6 +251,36 @@ def test_full_size_conservation(torch_gpu, precision, nx, ny):
     assert np.isfinite(divmax)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_device_diagnostics_match_host(torch_gpu, precision):
+    """Mass, energy, divB, integral / threshold mixing width and mixedness
+    reduced on the device equal the NumPy restatement of solver.py:335-444
+    on the downloaded state (1e-12 relative; the threshold width exactly),
+    on a developed magnetic-sieve state; the energy drift is 0 at the first
+    call and matches the host formula of solver.py:507-511 later."""
+    from amhd_diag import diagnostics
+    from amhd_ics import magnetic_sieve
+    from physics import prim_to_cons
+    from solver import MHDSolver
+
+    cfg, W = magnetic_sieve(nx=384, ny=192, seed=4, precision=precision, max_steps=10 ** 6)
+    s = MHDSolver(cfg)
+    s.U = prim_to_cons(W, cfg.gamma)
+    d0 = s.device_diagnostics()
+    assert d0["energy_drift_raw_pct"] == 0.0
+    s.advance(150)
+    d = s.device_diagnostics()
+    h = diagnostics(s.ctx.download_numpy(), cfg)
+    say(f"diag {precision}: mw {d['mw_integral']:.6e} / {h['mw_integral']:.6e}, thresh {d['mw_thresh']}, "
+        f"mixedness {d['mixedness']:.6f} / {h['mixedness']:.6f}")
+    for k in ("mass_total", "energy_total", "divB_max", "divB_L2", "mw_integral", "mixedness"):
+        assert d[k] == pytest.approx(h[k], rel=1e-12, abs=1e-300), k
+    assert d["mw_thresh"] == h["mw_thresh"]
+    assert 0.0 < d["mw_integral"] and 0.0 < d["mixedness"] < 1.0
+    e0, e1, bf = d0["energy_total"], d["energy_total"], s._cumulative_boundary_energy
+    assert d["energy_drift_corrected_pct"] == pytest.approx(abs(e1 - bf - e0) / abs(e0) * 100, rel=1e-12)
+
+
 # --------------------------------------------------------------------- V&V
 def test_vv_suite_gpu(torch_gpu):
     """The reference's verification suite on the HIP path with its own pass
