@@ -34,7 +34,7 @@ difference.  Every corrected step is then checked (density and pressure positive
 max|divB| not above 1.5x its pre-step value plus a floor); a failing step
 is discarded (amhd_rewind) and redone by the plain coarse solver.
 
-The fine-grid solver judges the result: integral mixing width and L1
+The fine-grid solver is the reference: integral mixing width and L1
 density error of the coarse run against the restricted fine run.
 """
 from __future__ import annotations
@@ -625,7 +625,7 @@ def cfg_cells(s) -> int:
     return s.cfg.nx * s.cfg.ny
 
 
-# ------------------------------------------------------------------- judge
+# ------------------------------------------------------- evaluation
 VARIANTS = {
     "closure": dict(keep=0.5, fields=None),          # the model's own fields
     "all": dict(keep=0.5, fields=list(range(N_IN))),
@@ -636,8 +636,8 @@ VARIANTS = {
 }
 
 
-def judge(seeds, model, steps: int = 2000, nx: int = 2048, ny: int = 1024, log=None, variants=("closure",)):
-    """Fine run (the judge) vs the plain coarse solver and closure variants
+def evaluate(seeds, model, steps: int = 2000, nx: int = 2048, ny: int = 1024, log=None, variants=("closure",)):
+    """Fine run (the reference) vs the plain coarse solver and closure variants
     (VARIANTS: limiter on/off, which corrections apply) from the restricted
     initial condition to the fine run's final time.  Errors are taken
     against the restricted fine state (the coarse grid's best attainable
@@ -686,7 +686,7 @@ def judge(seeds, model, steps: int = 2000, nx: int = 2048, ny: int = 1024, log=N
             rec[f"fallbacks_{tag}"] = cs.fallbacks
         rows.append(rec)
         if log:
-            log(f"[closure] judge seed {seed}: " + ", ".join(f"{k}={v:.4g}" if isinstance(v, float) else f"{k}={v}"
+            log(f"[closure] evaluation seed {seed}: " + ", ".join(f"{k}={v:.4g}" if isinstance(v, float) else f"{k}={v}"
                                                              for k, v in rec.items()))
     return rows
 
@@ -713,7 +713,7 @@ def fp64_fine_throughput(seed: int, steps: int = 2000, nx: int = 2048, ny: int =
 
 
 def summarize(rows, tags) -> dict:
-    """Means over the judged seeds."""
+    """Means over the evaluated seeds."""
     out = {}
     for t in tags:
         for k in ("mw_err", "l1rho", "fallbacks", "steps", "wall"):

@@ -2,8 +2,8 @@
 mixedness, interface mode amplitudes, enstrophy, stagnation pressure,
 div B and total energy, in float64 NumPy.
 
-``mixing_width`` is the integral mixing width alone (used as the judge of
-the learned closure).
+``mixing_width`` is the integral mixing width alone (the measure the
+learned closure is evaluated by).
 """
 from __future__ import annotations
 

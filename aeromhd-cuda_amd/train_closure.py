@@ -1,18 +1,18 @@
-"""Train and judge the learned flux closure on the GPU.
+"""Train and evaluate the learned flux closure on the GPU.
 
     python aeromhd-cuda_amd/train_closure.py --fields 8 --train 200 --test 200:220 \
         --iters 3000 --out aeromhd-cuda_amd/closure_weights.npz --report closure_report.json
 
 (the recipe of the shipped weights: the mixing flux alone, no aggregation
-rounds), and to judge given weights only:
+rounds), and to evaluate given weights only:
 
-    python aeromhd-cuda_amd/train_closure.py --judge-only aeromhd-cuda_amd/closure_weights.npz
+    python aeromhd-cuda_amd/train_closure.py --evaluate-only aeromhd-cuda_amd/closure_weights.npz
 
 Data: fine 2048x1024 magnetic-sieve runs (FP32, 2000 steps) for seeds
 0..train-1 restricted 8x to 256x128; held-out seeds for the validation
-loss and the fine-grid judge.  After the first fit, each aggregation round
+loss and the fine-grid evaluation.  After the first fit, each aggregation round
 adds pairs taken along the corrected coarse solver's own trajectories
-(rollout_pairs) and trains further on the union.  The judge also times the plain FP64
+(rollout_pairs) and trains further on the union.  The evaluation also times the plain FP64
 fine-grid solve of one seed for the speed comparison."""
 from __future__ import annotations
 
@@ -29,14 +29,14 @@ def log(*a):
     print(*a, flush=True)
 
 
-def final_judge(rep, model, judge_seeds, variants, args):
-    """Judge on the held-out seeds (with the limiter switched off as an
+def final_evaluation(rep, model, eval_seeds, variants, args):
+    """Evaluate on the held-out seeds (with the limiter switched off as an
     ablation), time the FP64 fine solve, write the report."""
     import amhd_closure as cl
 
-    rows = cl.judge(judge_seeds, model, steps=args.fine_steps, log=log, variants=variants + ("nolimit",))
-    rep["judge"] = rows
-    rep["fp64_fine"] = cl.fp64_fine_throughput(judge_seeds[0], args.fine_steps)
+    rows = cl.evaluate(eval_seeds, model, steps=args.fine_steps, log=log, variants=variants + ("nolimit",))
+    rep["evaluation"] = rows
+    rep["fp64_fine"] = cl.fp64_fine_throughput(eval_seeds[0], args.fine_steps)
     fine_cells = 2048 * 1024 * args.fine_steps
     tags = ("plain",) + variants + ("nolimit",)
     summ = cl.summarize(rows, tags)
@@ -57,16 +57,16 @@ def main():
     ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--fine-steps", type=int, default=2000)
-    ap.add_argument("--judge", default="")
+    ap.add_argument("--eval-seeds", default="", help="seed range a:b evaluated at the end (default: the test seeds)")
     ap.add_argument("--dagger-rounds", type=int, default=0)
     ap.add_argument("--dagger-seeds", default="0:40")
     ap.add_argument("--dagger-iters", type=int, default=2000)
     ap.add_argument("--dagger-lr", type=float, default=3e-4)
     ap.add_argument("--pullback", type=float, default=0.1)
-    ap.add_argument("--quick-judge", type=int, default=4, help="held-out seeds judged after every phase")
+    ap.add_argument("--quick-eval", type=int, default=4, help="held-out seeds evaluated after every phase")
     ap.add_argument("--variants", default="closure")
     ap.add_argument("--fields", default="8", help="network fields the closure corrects (8 = the mixing flux)")
-    ap.add_argument("--judge-only", default="", help="judge these weights on the test seeds; no training")
+    ap.add_argument("--evaluate-only", default="", help="evaluate these weights on the test seeds; no training")
     ap.add_argument("--out", default=os.path.join(os.path.dirname(os.path.abspath(__file__)), "closure_weights.npz"))
     ap.add_argument("--report", default="closure_report.json")
     args = ap.parse_args()
@@ -74,12 +74,12 @@ def main():
 
     a, b = (int(v) for v in args.test.split(":"))
     test_seeds = list(range(a, b))
-    judge_seeds = test_seeds if not args.judge else list(range(*(int(v) for v in args.judge.split(":"))))
+    eval_seeds = test_seeds if not args.eval_seeds else list(range(*(int(v) for v in args.eval_seeds.split(":"))))
     variants = tuple(args.variants.split(","))
-    if args.judge_only:
-        model = cl.FluxCNN.load(args.judge_only)
-        rep = {"weights": args.judge_only, "fields": cl.applied_fields(model.stats)}
-        final_judge(rep, model, judge_seeds, variants, args)
+    if args.evaluate_only:
+        model = cl.FluxCNN.load(args.evaluate_only)
+        rep = {"weights": args.evaluate_only, "fields": cl.applied_fields(model.stats)}
+        final_evaluation(rep, model, eval_seeds, variants, args)
         return
     t0 = time.time()
     data = cl.generate_pairs(range(args.train), args.pairs, steps=args.fine_steps, log=log)
@@ -96,9 +96,9 @@ def main():
 
     def quick(tag):
         model.save(args.out.replace(".npz", f".{tag.replace(' ', '')}.npz"))
-        if args.quick_judge <= 0:
+        if args.quick_eval <= 0:
             return
-        rows = cl.judge(test_seeds[:args.quick_judge], model, steps=args.fine_steps, variants=variants)
+        rows = cl.evaluate(test_seeds[:args.quick_eval], model, steps=args.fine_steps, variants=variants)
         ph = {"phase": tag, **cl.summarize(rows, ("plain",) + variants)}
         phases.append(ph)
         log(f"[closure] phase {tag}: " + ", ".join(f"{k}={v:.4g}" if isinstance(v, float) else f"{k}={v}"
@@ -125,7 +125,7 @@ def main():
            "history": hist, "phases": phases, "pullback": args.pullback, "holdout_loss": cl.eval_loss(model, test, dx, dy),
            "holdout_loss_plain": cl.baseline_loss(test[1], model.stats)}
     log(f"[closure] holdout loss {rep['holdout_loss']:.4f} vs plain {rep['holdout_loss_plain']:.4f}")
-    final_judge(rep, model, judge_seeds, variants, args)
+    final_evaluation(rep, model, eval_seeds, variants, args)
     log(f"[closure] wrote {args.out} and {args.report}")
 
 

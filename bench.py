@@ -178,13 +178,13 @@ def slab_measure(n_slabs: int = 4, steps: int = 40, warmup: int = 3, seed: int =
 def closure_measure(seed: int = 210, fine_steps: int = 2000):
     """The learned closure on one held-out magnetic-sieve seed: coarse
     256x128 FP32 with the shipped weights vs the plain coarse solver, both
-    judged against the fine 2048x1024 run restricted to the coarse grid, and
+    compared with the fine 2048x1024 run restricted to the coarse grid, and
     the closure's fine-equivalent cell-update rate over the plain FP64 fine
     solve's."""
     import amhd_closure as cl
 
     model = cl.FluxCNN.load(os.path.join(PKG, "closure_weights.npz"))
-    r = cl.judge([seed], model, steps=fine_steps)[0]
+    r = cl.evaluate([seed], model, steps=fine_steps)[0]
     fp64 = cl.fp64_fine_throughput(seed, fine_steps)
     fine_cells = 2048 * 1024 * fine_steps
     return {"workload": f"learned closure, magnetic sieve seed {seed} (held out): coarse 256x128 FP32 + BF16-MFMA "

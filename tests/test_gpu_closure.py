@@ -1,7 +1,7 @@
 """Learned closure on the GPU: MFMA lane layout, every convolution pass
 against the CPU oracle, whole-network gradients against the oracle chain,
 and (with the committed trained weights) the gain over the plain coarse
-solver as judged by the fine-grid solver."""
+solver as measured against the fine-grid solver."""
 import ctypes as C
 import os
 
@@ -202,7 +202,7 @@ def test_closure_glue_vs_oracle(torch_gpu, dtype):
 def test_closure_gain_over_plain_coarse(torch_gpu):
     """The shipped network (closure_weights.npz, which applies the mixing
     flux alone) against the plain coarse solver on two held-out seeds,
-    judged by the fine 2048x1024 solver restricted to the coarse grid: the
+    measured against the fine 2048x1024 solver restricted to the coarse grid: the
     mixing-width error at most half the plain solver's, the density (which
     the mixing flux cannot change) as the plain solver's, no fallbacks."""
     import amhd_closure as cl
@@ -211,7 +211,7 @@ def test_closure_gain_over_plain_coarse(torch_gpu):
     assert os.path.exists(path), "closure_weights.npz (trained weights) missing"
     model = cl.FluxCNN.load(path)
     assert cl.applied_fields(model.stats) == [cl.RC]
-    rows = cl.judge([210, 217], model, log=print)
+    rows = cl.evaluate([210, 217], model, log=print)
     for r in rows:
         assert r["mw_err_closure"] < 0.5 * r["mw_err_plain"], r
         assert r["l1rho_closure"] == pytest.approx(r["l1rho_plain"], rel=1e-6), r
