@@ -509,10 +509,12 @@ class ClosureSolver:
     runs on the context's stream; the state is read in place."""
 
     def __init__(self, cfg, model: FluxCNN | None, divb_floor: float = 1.0, keep: float | None = 0.5,
-                 fields=None):
+                 fields=None, gain: float | None = None):
         """fields: the network fields (positions in FEAT) whose corrections
         are applied, None for the model's own (all unless it was trained
-        on fewer); the others keep the plain solver's fluxes."""
+        on fewer); the others keep the plain solver's fluxes.  gain: a
+        factor on the corrections, None for the model's calibrated one
+        (stats["gain"], 1 if absent)."""
         from solver import MHDSolver
 
         self.s = MHDSolver(cfg)
@@ -525,6 +527,9 @@ class ClosureSolver:
         if fields is not None:
             self.mask[:] = 0.0
             self.mask[list(fields)] = 1.0
+        if gain is None:
+            gain = float(np.asarray(model.stats.get("gain", 1.0))) if model is not None else 1.0
+        self.mask *= np.float32(gain)
         self.fallbacks = 0
         self.limited = 0.0           # summed fraction of cells with theta < 1
         self._bufs = None
@@ -559,19 +564,22 @@ class ClosureSolver:
         if self._bufs is None:
             self._alloc()
         cfg, ctx = self.s.cfg, self.s.ctx
-        x = features([self._state_ptr()], ctx.dtype, FEAT, cfg.nx, cfg.ny, self.model.stats, self.x, self.stream)
+        self.pre = self._state_ptr()       # stays intact through the step (it is the rewind target)
+        x = features([self.pre], ctx.dtype, FEAT, cfg.nx, cfg.ny, self.model.stats, self.x, self.stream)
         return self.model.forward(x, stream=self.stream, bufs=self.outs)
 
     def correct(self, dt: float):
         """U += dt * S of the predicted corrections on the current (post-step)
-        state, limited against that state (asynchronous)."""
+        state, limited against that state; the concentration bounds also
+        take in the pre-step state (asynchronous)."""
         cfg, ctx, st = self.s.cfg, self.s.ctx, self.model.stats
         u = (C.c_void_p * 1)(self._state_ptr())
         lim = self.keep is not None
         _L.check(ctx.lib.amhd_closure_source(_p(self.outs[-1]), u, u, ctx.dtype, _int9(FEAT), 1, cfg.nx, cfg.ny,
                                              _f9(st["f_scale"] * self.mask), float(cfg.dx), float(cfg.dy), float(dt),
                                              float(cfg.gamma), float(self.keep) if lim else 0.0,
-                                             _p(self.theta) if lim else None, 1, self.stream),
+                                             _p(self.theta) if lim else None,
+                                             (C.c_void_p * 1)(self.pre) if lim else None, 1, self.stream),
                  "amhd_closure_source")
 
     def begin(self):
@@ -636,14 +644,16 @@ VARIANTS = {
 }
 
 
-def evaluate(seeds, model, steps: int = 2000, nx: int = 2048, ny: int = 1024, log=None, variants=("closure",)):
+def evaluate(seeds, model, steps: int = 2000, nx: int = 2048, ny: int = 1024, log=None, variants=("closure",),
+             specs=None):
     """Fine run (the reference) vs the plain coarse solver and closure variants
     (VARIANTS: limiter on/off, which corrections apply) from the restricted
     initial condition to the fine run's final time.  Errors are taken
     against the restricted fine state (the coarse grid's best attainable
     answer): the integral mixing width of the restricted state (mw_err_*)
     and the L1 density error (l1rho_*); the fine grid's own mixing width is
-    reported as mw_fine."""
+    reported as mw_fine.  specs: {tag: ClosureSolver keywords} in place of
+    VARIANTS."""
     torch = _L.require_gpu()
     from amhd_diag import mixing_width
 
@@ -667,7 +677,8 @@ def evaluate(seeds, model, steps: int = 2000, nx: int = 2048, ny: int = 1024, lo
         rho_f = Ufr[0]
         mw_fr = mixing_width(ghosted(Ufr).cpu().numpy(), ccfg)
         rec = {"seed": seed, "t_end": t_end, "mw_fine": mw_f, "mw_fine_restricted": mw_fr, "fine_wall_s": fine_wall}
-        runs = [("plain", None, {})] + [(v, model, VARIANTS[v]) for v in variants]
+        table = specs if specs is not None else VARIANTS
+        runs = [("plain", None, {})] + [(v, model, table[v]) for v in variants]
         for tag, m, kw in runs:
             cs = ClosureSolver(ccfg, m, **kw)
             cs.s.U = ghosted(fine["U0"])
@@ -710,6 +721,22 @@ def fp64_fine_throughput(seed: int, steps: int = 2000, nx: int = 2048, ny: int =
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     return {"mcups": nx * ny * n / wall / 1e6, "wall_s_for_run": wall * steps / n, "steps": steps}
+
+
+def calibrate_gain(model, seeds, gains, steps: int = 2000, log=None):
+    """The correction gain that minimises the mean mixing-width error over
+    `seeds` (training seeds: the held-out ones stay untouched); a network
+    fitted by least squares under-predicts sharp corrections, and a scalar
+    gain is the one-parameter remedy.  Stores it in model.stats["gain"]."""
+    specs = {f"g{g:g}": dict(keep=0.5, gain=float(g)) for g in gains}
+    rows = evaluate(seeds, model, steps=steps, variants=tuple(specs), specs=specs)
+    err = {t: sum(r[f"mw_err_{t}"] for r in rows) / len(rows) for t in specs}
+    best = min(err, key=err.get)
+    model.stats["gain"] = np.asarray(specs[best]["gain"], dtype=np.float32)
+    if log:
+        log(f"[closure] gain calibration on seeds {list(seeds)}: " +
+            ", ".join(f"{t}={e:.4f}" for t, e in err.items()) + f" -> {best}")
+    return float(specs[best]["gain"]), err
 
 
 def summarize(rows, tags) -> dict:

@@ -83,7 +83,7 @@ SIGNATURES = {
     "amhd_closure_features": (_i, [C.POINTER(_vp), _i, C.POINTER(_i), _i, _i, _i, C.POINTER(C.c_float),
                                    C.POINTER(C.c_float), _vp, _vp]),
     "amhd_closure_source": (_i, [_vp, C.POINTER(_vp), C.POINTER(_vp), _i, C.POINTER(_i), _i, _i, _i,
-                                 C.POINTER(C.c_float), _d, _d, _d, _d, _d, _vp, _i, _vp]),
+                                 C.POINTER(C.c_float), _d, _d, _d, _d, _d, _vp, C.POINTER(_vp), _i, _vp]),
 }
 
 _lib = None

@@ -451,13 +451,15 @@ __device__ __forceinline__ void limited_fluxes(const FaceCorr& fc, const FluxSca
 // donor-cell rhoC transport applied (rho1, rc1), the smallest over the
 // faces of the largest t in {1, ..., 1/32, 0} for which rc1 + 4 t G_face
 // alone stays within [C_lo rho1, C_hi rho1], C_lo / C_hi the min / max of C
-// over the cell and its face neighbours.  The set is convex and, by the
-// donor-cell form, holds rc1 itself, so any face weights up to theta_mix
-// keep C within the bounds.
+// over the cell and its face neighbours -- in the state and, when `bst`
+// is given, in a second state too (the state before the step: Zalesak's
+// bounds).  The set is convex and, by the donor-cell form, holds rc1
+// itself, so any face weights up to theta_mix keep C within the bounds.
 template <class real>
 __global__ __launch_bounds__(256) void closure_theta_mix_kernel(const float* __restrict__ dF, ConstPtrs st,
-                                                                FluxScale fs, int n, int nx, int ny, double dtdx,
-                                                                double dtdy, const float* __restrict__ theta,
+                                                                ConstPtrs bst, bool has_b, FluxScale fs, int n,
+                                                                int nx, int ny, double dtdx, double dtdy,
+                                                                const float* __restrict__ theta,
                                                                 float* __restrict__ theta_mix) {
     const long plane = (long)nx * ny;
     const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -475,8 +477,13 @@ __global__ __launch_bounds__(256) void closure_theta_mix_kernel(const float* __r
     limited_fluxes(fc, fs, w, zero, q, F);
     const double rho1 = double(u[c]) + dtdx * (F.xl[FRHO] - F.xr[FRHO]) + dtdy * (F.yl[FRHO] - F.yr[FRHO]);
     const double rc1 = double(u[9 * plane + c]) + dtdx * (F.xl[FRC] - F.xr[FRC]) + dtdy * (F.yl[FRC] - F.yr[FRC]);
-    const double clo = fmin(q.c, fmin(fmin(q.xl, q.xr), fmin(q.yl, q.yr)));
-    const double chi = fmax(q.c, fmax(fmax(q.xl, q.xr), fmax(q.yl, q.yr)));
+    double clo = fmin(q.c, fmin(fmin(q.xl, q.xr), fmin(q.yl, q.yr)));
+    double chi = fmax(q.c, fmax(fmax(q.xl, q.xr), fmax(q.yl, q.yr)));
+    if (has_b) {
+        const Nbr b = neighbours((const real*)bst.p[m], plane, c, i, j, nx, ny);
+        clo = fmin(clo, fmin(b.c, fmin(fmin(b.xl, b.xr), fmin(b.yl, b.yr))));
+        chi = fmax(chi, fmax(b.c, fmax(fmax(b.xl, b.xr), fmax(b.yl, b.yr))));
+    }
     const double lo = clo * rho1, hi = chi * rho1, sc = fs.s[FRC];
     auto largest = [&](double g) -> double {
         double t = 1.0;
@@ -654,7 +661,7 @@ int amhd_closure_features(const void* const* states, int dtype, const int* field
 
 int amhd_closure_source(const float* dF, const void* const* states, void* const* srcs, int dtype, const int* fields,
                         int n, int nx, int ny, const float* fscale, double dx, double dy, double dt, double gamma,
-                        double keep, float* theta, int accumulate, void* stream) {
+                        double keep, float* theta, const void* const* bounds, int accumulate, void* stream) {
     if (!dF || !states || !srcs || !fields || !fscale || n <= 0 || nx <= 0 || ny <= 0 || !(dx > 0) || !(dy > 0))
         return cfail(-1, "amhd_closure_source: bad argument");
     if (dtype != AMHD_F64 && dtype != AMHD_F32) return cfail(-1, "amhd_closure_source: bad dtype %d", dtype);
@@ -672,13 +679,16 @@ int amhd_closure_source(const float* dF, const void* const* states, void* const*
     hipStream_t s = (hipStream_t)stream;
     for (int m0 = 0; m0 < n; m0 += CHUNK) {
         const int cnt = n - m0 < CHUNK ? n - m0 : CHUNK;
-        ConstPtrs st{};
+        ConstPtrs st{}, bs{};
         MutPtrs sp{};
         for (int m = 0; m < cnt; ++m) {
-            if (!srcs[m0 + m] || !states[m0 + m]) return cfail(-1, "amhd_closure_source: null pointer %d", m0 + m);
+            if (!srcs[m0 + m] || !states[m0 + m] || (bounds && !bounds[m0 + m]))
+                return cfail(-1, "amhd_closure_source: null pointer %d", m0 + m);
             sp.p[m] = srcs[m0 + m];
             st.p[m] = states[m0 + m];
+            bs.p[m] = bounds ? bounds[m0 + m] : nullptr;
         }
+        const bool hb = bounds != nullptr;
         const long tot = (long)cnt * plane;
         const float* d = dF + (long)m0 * plane * 32;
         float* th = theta ? theta + (long)m0 * plane : nullptr;
@@ -687,10 +697,12 @@ int amhd_closure_source(const float* dF, const void* const* states, void* const*
         if (th) {
             if (dtype == AMHD_F64) {
                 closure_theta_kernel<double><<<g, 256, 0, s>>>(d, st, fs, cnt, nx, ny, dt / dx, dt / dy, gamma, keep, th);
-                closure_theta_mix_kernel<double><<<g, 256, 0, s>>>(d, st, fs, cnt, nx, ny, dt / dx, dt / dy, th, tm);
+                closure_theta_mix_kernel<double><<<g, 256, 0, s>>>(d, st, bs, hb, fs, cnt, nx, ny, dt / dx, dt / dy, th,
+                                                                   tm);
             } else {
                 closure_theta_kernel<float><<<g, 256, 0, s>>>(d, st, fs, cnt, nx, ny, dt / dx, dt / dy, gamma, keep, th);
-                closure_theta_mix_kernel<float><<<g, 256, 0, s>>>(d, st, fs, cnt, nx, ny, dt / dx, dt / dy, th, tm);
+                closure_theta_mix_kernel<float><<<g, 256, 0, s>>>(d, st, bs, hb, fs, cnt, nx, ny, dt / dx, dt / dy, th,
+                                                                  tm);
             }
             if (hipGetLastError() != hipSuccess) return cfail(-2, "amhd_closure_source: limiter launch failed");
         }

@@ -114,33 +114,24 @@ __device__ __forceinline__ void load_cell(const real* __restrict__ base, long pl
 // FP64 CFL contributions of one cell (solver.py:209-214), from conserved u:
 // m[0] collects cf^2 = gamma p/rho + B^2/rho (sqrt is monotone, so
 // sqrt(max cf^2) == max cf; the square root is taken once per step),
-// m[1] |vx|+|vy|, m[2] |vx|, m[3] |vy|.  FP64 state: the reference's
-// expression; FP32 state: (gamma p + B^2)/rho in FP32, maxima in FP64.
+// m[1] |vx|+|vy|, m[2] |vx|, m[3] |vy|, all in FP64 (an FP32 state is
+// widened first).  FP64 state: the reference's expression; FP32 state:
+// (gamma p + B^2)/rho.
 template <class real>
 __device__ __forceinline__ void dt_terms(const real (&u)[NV], double gamma, double (&m)[NPART_DT]) {
-    if constexpr (sizeof(real) == sizeof(double)) {
-        // FP64 state: the reference's expressions (solver.py:208-214)
-        double w[NV];
-        cons_to_prim<double, ExactMath<double>>(u, w, gamma, false);
-        const double b2 = w[iBX] * w[iBX] + w[iBY] * w[iBY] + w[iBZ] * w[iBZ];
-        const double cf2 = gamma * w[iPR] / w[iRHO] + b2 / w[iRHO];
-        const double ax = fabs(w[iVX]), ay = fabs(w[iVY]);
-        m[0] = fmax(m[0], cf2);
-        m[1] = fmax(m[1], ax + ay);
-        m[2] = fmax(m[2], ax);
-        m[3] = fmax(m[3], ay);
-    } else {
-        // FP32 state: FP32 terms (cf^2 = (gamma p + B^2) / rho), FP64 maxima
-        float w[NV];
-        cons_to_prim<float, FastMath<float>>(u, w, float(gamma), false);
-        const float b2 = w[iBX] * w[iBX] + w[iBY] * w[iBY] + w[iBZ] * w[iBZ];
-        const float cf2 = (float(gamma) * w[iPR] + b2) * __builtin_amdgcn_rcpf(w[iRHO]);
-        const float ax = fabsf(w[iVX]), ay = fabsf(w[iVY]);
-        m[0] = fmax(m[0], double(cf2));
-        m[1] = fmax(m[1], double(ax + ay));
-        m[2] = fmax(m[2], double(ax));
-        m[3] = fmax(m[3], double(ay));
-    }
+    double ud[NV], w[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) ud[v] = double(u[v]);
+    cons_to_prim<double, ExactMath<double>>(ud, w, gamma, false);
+    const double b2 = w[iBX] * w[iBX] + w[iBY] * w[iBY] + w[iBZ] * w[iBZ];
+    double cf2;
+    if constexpr (sizeof(real) == sizeof(double)) cf2 = gamma * w[iPR] / w[iRHO] + b2 / w[iRHO];
+    else cf2 = (gamma * w[iPR] + b2) / w[iRHO];
+    const double ax = fabs(w[iVX]), ay = fabs(w[iVY]);
+    m[0] = fmax(m[0], cf2);
+    m[1] = fmax(m[1], ax + ay);
+    m[2] = fmax(m[2], ax);
+    m[3] = fmax(m[3], ay);
 }
 
 // Boundary energy-flux density of one cell (solver.py:233-239), FP64.

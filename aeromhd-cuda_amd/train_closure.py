@@ -4,9 +4,10 @@
         --iters 3000 --out aeromhd-cuda_amd/closure_weights.npz --report closure_report.json
 
 (the recipe of the shipped weights: the mixing flux alone, no aggregation
-rounds), and to evaluate given weights only:
+rounds, the correction gain calibrated on training seeds 0-7), and to
+evaluate given weights (calibrating them first unless --calibrate ''):
 
-    python aeromhd-cuda_amd/train_closure.py --evaluate-only aeromhd-cuda_amd/closure_weights.npz
+    python aeromhd-cuda_amd/train_closure.py --evaluate-only W.npz --out W_calibrated.npz
 
 Data: fine 2048x1024 magnetic-sieve runs (FP32, 2000 steps) for seeds
 0..train-1 restricted 8x to 256x128; held-out seeds for the validation
@@ -67,6 +68,8 @@ def main():
     ap.add_argument("--variants", default="closure")
     ap.add_argument("--fields", default="8", help="network fields the closure corrects (8 = the mixing flux)")
     ap.add_argument("--evaluate-only", default="", help="evaluate these weights on the test seeds; no training")
+    ap.add_argument("--calibrate", default="0:8", help="training seeds a:b for the gain calibration ('' to skip)")
+    ap.add_argument("--gains", default="1.0,1.1,1.2,1.3,1.4,1.5")
     ap.add_argument("--out", default=os.path.join(os.path.dirname(os.path.abspath(__file__)), "closure_weights.npz"))
     ap.add_argument("--report", default="closure_report.json")
     args = ap.parse_args()
@@ -76,9 +79,20 @@ def main():
     test_seeds = list(range(a, b))
     eval_seeds = test_seeds if not args.eval_seeds else list(range(*(int(v) for v in args.eval_seeds.split(":"))))
     variants = tuple(args.variants.split(","))
+    gains = [float(g) for g in args.gains.split(",")]
+
+    def calibrate(model, rep):
+        if not args.calibrate:
+            return
+        ca, cb = (int(v) for v in args.calibrate.split(":"))
+        g, err = cl.calibrate_gain(model, range(ca, cb), gains, steps=args.fine_steps, log=log)
+        rep["calibration"] = {"seeds": args.calibrate, "mw_err_by_gain": err, "gain": g}
+        model.save(args.out)
+
     if args.evaluate_only:
         model = cl.FluxCNN.load(args.evaluate_only)
         rep = {"weights": args.evaluate_only, "fields": cl.applied_fields(model.stats)}
+        calibrate(model, rep)
         final_evaluation(rep, model, eval_seeds, variants, args)
         return
     t0 = time.time()
@@ -125,6 +139,7 @@ def main():
            "history": hist, "phases": phases, "pullback": args.pullback, "holdout_loss": cl.eval_loss(model, test, dx, dy),
            "holdout_loss_plain": cl.baseline_loss(test[1], model.stats)}
     log(f"[closure] holdout loss {rep['holdout_loss']:.4f} vs plain {rep['holdout_loss_plain']:.4f}")
+    calibrate(model, rep)
     final_evaluation(rep, model, eval_seeds, variants, args)
     log(f"[closure] wrote {args.out} and {args.report}")
 

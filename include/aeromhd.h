@@ -249,14 +249,17 @@ int amhd_mfma_probe(const float* A, const float* B, float* D, void* stream);
  * theta_mix_c.  theta_c = min over the cell's faces of the largest t in
  * {1, 1/2, ..., 1/32, 0} for which states + 4 t dt S_face alone keeps rho
  * and p at or above keep times their values; theta_mix_c likewise keeps C
- * within the min/max of C over the cell and its face neighbours.  By
+ * within the min/max of C over the cell and its face neighbours, taken in
+ * states and, if bounds (n device addresses, or NULL) is given, in those
+ * states as well (the states before the step, for a post-step
+ * correction: Zalesak's bounds).  By
  * convexity every cell of states + dt*S keeps both bounds, and S stays a
  * flux difference, so it conserves exactly. */
 int amhd_closure_features(const void* const* states, int dtype, const int* fields, int n, int nx, int ny,
                           const float* mean, const float* std, void* x, void* stream);
 int amhd_closure_source(const float* dF, const void* const* states, void* const* srcs, int dtype, const int* fields,
                         int n, int nx, int ny, const float* fscale, double dx, double dy, double dt, double gamma,
-                        double keep, float* theta, int accumulate, void* stream);
+                        double keep, float* theta, const void* const* bounds, int accumulate, void* stream);
 
 #ifdef __cplusplus
 }

@@ -298,12 +298,13 @@ def local_bounds(a):
     return lo, hi
 
 
-def limiter_theta_mix(U0, dF, theta, fscale, dt, dx, dy):
+def limiter_theta_mix(U0, dF, theta, fscale, dt, dx, dy, Ub=None):
     """Per-cell limiter of the mixing flux G: with the mass flux limited by
     theta's face weights and its donor-cell rhoC transport applied, the
     smallest over the faces of the largest t of THETAS (else 0) for which
     the face's G alone, times 4 t, keeps C = rhoC/rho within the min and max
-    of C over the cell and its face neighbours in U0."""
+    of C over the cell and its face neighbours in U0 and, if given, in Ub
+    as well (the state before the step: Zalesak's bounds)."""
     U0 = np.asarray(U0, dtype=np.float64)
     n9 = len(FEAT)
     dF = np.asarray(dF, dtype=np.float64)
@@ -320,6 +321,9 @@ def limiter_theta_mix(U0, dF, theta, fscale, dt, dx, dy):
     rho1 = U0[:, 0] + dt * Sm[..., 0]
     rc1 = U0[:, 9] + dt * Sm[..., RC]
     clo, chi = local_bounds(C)
+    if Ub is not None:
+        blo, bhi = local_bounds(concentration(np.asarray(Ub, dtype=np.float64)))
+        clo, chi = np.minimum(clo, blo), np.maximum(chi, bhi)
     th = np.ones(theta.shape)
     for G in _faces(fx[..., RC], dt, dx, dy, fy[..., RC]):
         th = np.minimum(th, _largest(lambda t: (rc1 + 4.0 * t * G >= clo * rho1) & (rc1 + 4.0 * t * G <= chi * rho1)))

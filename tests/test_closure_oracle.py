@@ -213,6 +213,14 @@ def test_mixing_limiter_keeps_concentration_bounds():
 
     c1 = c_after(tm)
     assert np.all(c1 >= lo - 1e-12) and np.all(c1 <= hi + 1e-12)
+    # with the bounds widened by a second state (the state before the step)
+    Ub = U0.copy()
+    Ub[:, 9] = U0[:, 0] * rng.uniform(0, 1, (2, 10, 8))
+    tb = co.limiter_theta_mix(U0, dF, th, fs, dt, dx, dy, Ub)
+    assert np.all(tb >= tm) and np.any(tb > tm)
+    blo, bhi = co.local_bounds(co.concentration(Ub))
+    cb = c_after(tb)
+    assert np.all(cb >= np.minimum(lo, blo) - 1e-12) and np.all(cb <= np.maximum(hi, bhi) + 1e-12)
     c0 = c_after(np.ones_like(tm))
     assert np.any(c0 < lo - 1e-6) or np.any(c0 > hi + 1e-6)
     # the donor-cell part alone (no mixing flux) keeps the bounds for any theta

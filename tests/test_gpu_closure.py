@@ -152,9 +152,9 @@ def test_closure_glue_vs_oracle(torch_gpu, dtype):
     sp = [src.data_ptr() + m * 10 * nx * ny * esz for m in range(n)]
     V = C.c_void_p * n
 
-    def run(th, out=sp, acc=0):
+    def run(th, out=sp, acc=0, bnd=None):
         L.check(lib.amhd_closure_source(_p(dF), V(*ptrs), V(*out), dtype, cl._int9(cl.FEAT), n, nx, ny, cl._f9(fs),
-                                        dx, dy, dt, gamma, keep, _p(th) if th is not None else None, acc,
+                                        dx, dy, dt, gamma, keep, _p(th) if th is not None else None, bnd, acc,
                                         L.stream_handle(torch)), "amhd_closure_source")
         torch.cuda.synchronize()
         return src.cpu().numpy().astype(np.float64)
@@ -182,6 +182,14 @@ def test_closure_glue_vs_oracle(torch_gpu, dtype):
     ref1 = co.source_from_flux(dFh[..., :18] * np.concatenate([fs64, fs64]), dx, dy, Ch)
     np.testing.assert_allclose(np.transpose(S1[:, cl.FEAT], (0, 2, 3, 1)), ref1, rtol=0,
                                atol=tol * np.abs(ref1).max())
+    # concentration bounds widened by a second state (the state before a step)
+    Ub = torch.tensor(_state(rng, n, nx, ny, gamma), dtype=tdt, device="cuda").contiguous()
+    bptrs = [Ub.data_ptr() + m * 10 * nx * ny * esz for m in range(n)]
+    run(theta, bnd=V(*bptrs))
+    tb_gpu = theta.cpu().numpy().astype(np.float64)[n * nx * ny:].reshape(n, nx, ny)
+    tb_ref = co.limiter_theta_mix(Uh, dFh, th_gpu, fs64, dt, dx, dy, Ub.cpu().numpy().astype(np.float64))
+    assert np.sum(tb_gpu != tb_ref) <= 1
+    assert np.all(tb_gpu >= tm_gpu) and np.any(tb_gpu > tm_gpu)
     # in place on the states: U += dt * S with the limiter taken on U itself
     run(theta, out=ptrs, acc=1)
     Unew = Ud.cpu().numpy().astype(np.float64)
