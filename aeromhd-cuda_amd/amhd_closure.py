@@ -635,12 +635,13 @@ def cfg_cells(s) -> int:
 
 # ------------------------------------------------------- evaluation
 VARIANTS = {
-    "closure": dict(keep=0.5, fields=None),          # the model's own fields
-    "all": dict(keep=0.5, fields=list(range(N_IN))),
-    "nolimit": dict(keep=None, fields=None),
-    "mix": dict(keep=0.5, fields=[RC]),
-    "mass": dict(keep=0.5, fields=[0, RC]),
-    "hydro": dict(keep=0.5, fields=[0, 1, 2, 3, 7, RC]),
+    "closure": dict(keep=0.5),                       # the model's own fields and gain
+    "nolimit": dict(keep=None),                      # ... without the limiters
+    # ablations of the network itself (gain 1)
+    "all": dict(keep=0.5, fields=list(range(N_IN)), gain=1.0),
+    "mix": dict(keep=0.5, fields=[RC], gain=1.0),
+    "mass": dict(keep=0.5, fields=[0, RC], gain=1.0),
+    "hydro": dict(keep=0.5, fields=[0, 1, 2, 3, 7, RC], gain=1.0),
 }
 
 
