@@ -154,15 +154,17 @@ __device__ __forceinline__ real vl_slope(real wm, real w0, real wp) {
 
 // Half the van Leer slope: the offset of the face states W +/- slope/2
 // (physics.py:329-336) from the cell value.  The fast path forms
-// ab * rcp(dm + dp + 1e-30) directly, which equals 0.5 * ((2 ab) * rcp(..))
-// bit for bit (scaling by 2 and by 1/2 is exact).
+// ab / (dm + dp) directly, which equals 0.5 * (2 ab / (..)) exactly (scaling
+// by 2 and by 1/2 is exact), and drops the 1e-30 guard: where ab > 0, dm
+// and dp are nonzero with one sign, so dm + dp != 0; where ab <= 0 the
+// quotient (possibly inf or NaN) is not selected.
 template <class real, class P>
 __device__ __forceinline__ real vl_half_d(real dm, real dp) {
     if constexpr (P::exact) {
         return real(0.5) * vl_slope_d<real, P>(dm, dp);
     } else {
         const real ab = dm * dp;
-        const real h = ab * P::rcp(dm + dp + real(1e-30));
+        const real h = ab * P::rcp(dm + dp);
         return ab > real(0) ? h : real(0);
     }
 }

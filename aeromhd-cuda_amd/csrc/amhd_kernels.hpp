@@ -116,18 +116,31 @@ __device__ __forceinline__ void load_cell(const real* __restrict__ base, long pl
 // sqrt(max cf^2) == max cf; the square root is taken once per step),
 // m[1] |vx|+|vy|, m[2] |vx|, m[3] |vy|, all in FP64 (an FP32 state is
 // widened first).  FP64 state: the reference's expression; FP32 state:
-// (gamma p + B^2)/rho.
+// (gamma p + B^2) * (1/rho).
 template <class real>
 __device__ __forceinline__ void dt_terms(const real (&u)[NV], double gamma, double (&m)[NPART_DT]) {
-    double ud[NV], w[NV];
-#pragma unroll
-    for (int v = 0; v < NV; ++v) ud[v] = double(u[v]);
-    cons_to_prim<double, ExactMath<double>>(ud, w, gamma, false);
-    const double b2 = w[iBX] * w[iBX] + w[iBY] * w[iBY] + w[iBZ] * w[iBZ];
-    double cf2;
-    if constexpr (sizeof(real) == sizeof(double)) cf2 = gamma * w[iPR] / w[iRHO] + b2 / w[iRHO];
-    else cf2 = (gamma * w[iPR] + b2) / w[iRHO];
-    const double ax = fabs(w[iVX]), ay = fabs(w[iVY]);
+    double cf2, ax, ay;
+    if constexpr (sizeof(real) == sizeof(double)) {
+        // the reference's expressions (solver.py:208-214)
+        double w[NV];
+        cons_to_prim<double, ExactMath<double>>(u, w, gamma, false);
+        const double b2 = w[iBX] * w[iBX] + w[iBY] * w[iBY] + w[iBZ] * w[iBZ];
+        cf2 = gamma * w[iPR] / w[iRHO] + b2 / w[iRHO];
+        ax = fabs(w[iVX]);
+        ay = fabs(w[iVY]);
+    } else {
+        // FP32 state widened to FP64; one IEEE division (1/rho), reused
+        const double rho = fmax(double(u[RHO]), FLOOR_RHO);
+        const double ri = 1.0 / rho;
+        const double vx = double(u[MX]) * ri, vy = double(u[MY]) * ri, vz = double(u[MZ]) * ri;
+        const double bx = double(u[BX]), by = double(u[BY]), bz = double(u[BZ]);
+        const double b2 = bx * bx + by * by + bz * bz;
+        const double p = fmax((gamma - 1.0) * (double(u[EN]) - 0.5 * rho * (vx * vx + vy * vy + vz * vz) - 0.5 * b2),
+                              FLOOR_PR);
+        cf2 = (gamma * p + b2) * ri;
+        ax = fabs(vx);
+        ay = fabs(vy);
+    }
     m[0] = fmax(m[0], cf2);
     m[1] = fmax(m[1], ax + ay);
     m[2] = fmax(m[2], ax);
