@@ -225,3 +225,47 @@ def test_closure_gain_over_plain_coarse(torch_gpu):
         assert r["mw_err_closure"] < 0.2 * r["mw_err_plain"], r
         assert r["l1rho_closure"] == pytest.approx(r["l1rho_plain"], rel=1e-6), r
         assert r["fallbacks_closure"] == 0, r
+
+
+def _coarse_sieve(seed):
+    from amhd_ics import cons_from_prim_host, magnetic_sieve
+
+    cfg, W = magnetic_sieve(nx=256, ny=128, precision="fp32", seed=seed, max_steps=10 ** 6)
+    cfg.t_end = 1e30
+    return cfg, cons_from_prim_host(W, cfg.gamma)
+
+
+def test_closure_fallback_rewinds_exactly(torch_gpu):
+    """A guard that rejects every corrected step (divB floor -1e30) makes
+    every step a rewind plus a plain step: the run then equals the plain
+    coarse solver's bit for bit, step for step."""
+    import amhd_closure as cl
+
+    model = cl.FluxCNN.load(os.path.join(PKG, "closure_weights.npz"))
+    cfg, U = _coarse_sieve(211)
+    runs = {}
+    for tag, m, floor in (("plain", None, 1.0), ("rejected", model, -1e30)):
+        cs = cl.ClosureSolver(cfg, m, divb_floor=floor)
+        cs.s.U = U.copy()
+        cs.run(1e30, max_steps=40)
+        runs[tag] = (cs.s.ctx.download_numpy(), cs.s.t, cs.s.step, cs.fallbacks)
+    assert runs["rejected"][3] == runs["rejected"][2] == 40
+    np.testing.assert_array_equal(runs["rejected"][0], runs["plain"][0])
+    assert runs["rejected"][1] == runs["plain"][1]
+
+
+def test_closure_guard_catches_bad_corrections(torch_gpu):
+    """Unlimited corrections of every field at 100x the calibrated scale
+    drive the pressure negative; the guard rejects those steps and the
+    state stays finite with positive density and pressure."""
+    import amhd_closure as cl
+
+    model = cl.FluxCNN.load(os.path.join(PKG, "closure_weights.npz"))
+    cfg, U = _coarse_sieve(212)
+    cs = cl.ClosureSolver(cfg, model, keep=None, fields=list(range(cl.N_IN)), gain=100.0)
+    cs.s.U = U
+    cs.run(1e30, max_steps=30)
+    rho_min, p_min, divb, nonfinite = cs.check()
+    print(f"fallbacks {cs.fallbacks}/{cs.s.step}, rho_min {rho_min:.3e}, p_min {p_min:.3e}", flush=True)
+    assert cs.fallbacks > 0
+    assert nonfinite == 0 and rho_min > 0 and p_min > 0
