@@ -30,9 +30,11 @@ amhd_closure_source through a positivity limiter: per cell the largest
 theta in {1, 1/2, .., 1/32, 0} for which the post-step state plus
 theta*dt*S keeps half of its density and pressure, and each face takes the
 smaller theta of its two cells, so the limited correction is still a flux
-difference.  Every corrected step is then checked (density and pressure positive, no non-finite values,
-max|divB| not above 1.5x its pre-step value plus a floor); a failing step
-is discarded (amhd_rewind) and redone by the plain coarse solver.
+difference.  The corrected state is checked before it is written
+(amhd_closure_check: density and pressure positive, no non-finite values,
+max|divB| not above 1.5x its pre-step value plus a floor); a correction
+that fails is not applied, and the plain coarse step stands.  BatchClosure
+does the same for many instances in one batched step.
 
 The fine-grid solver is the reference: integral mixing width and L1
 density error of the coarse run against the restricted fine run.
@@ -369,7 +371,6 @@ def rollout_pairs(model, seeds, starts: int = 10, horizon: int = 25, span: int =
                 Fk, Fk1 = F[k].double(), F[k + 1].double()
                 R.append(((Fk1 - Fk - (Uc - V) + pullback * (Fk - V)) / dt)[FEAT].float())
                 CP.append((Uc[9] / Uc[0]).float())
-                cs.s.t += dt
         del F
         if log:
             log(f"[closure] rollout seed {seed}: {len(X) - n0} pairs, {cs.fallbacks} fallbacks, "
@@ -502,11 +503,89 @@ def baseline_loss(R, stats) -> float:
 
 
 # ------------------------------------------------------------ coarse solver
+class _Corrector:
+    """The learned correction of n coarse states of one grid: network pass
+    on the pre-step states, limited source against the post-step states,
+    the guard evaluated on U + dt S before anything is written, and
+    U += dt S for the instances that pass (the others keep their plain
+    step: the fallback).  All launches on `stream`."""
+
+    def __init__(self, model: FluxCNN, n: int, cfg, dtype: int, stream, keep, divb_floor, fields, gain):
+        torch = _L.require_gpu()
+        self.model, self.n, self.cfg, self.dtype, self.stream = model, n, cfg, dtype, stream
+        self.keep, self.divb_floor = keep, divb_floor
+        if fields is None:
+            fields = applied_fields(model.stats)
+        self.mask = np.zeros(N_IN, dtype=np.float32)
+        self.mask[list(fields)] = 1.0
+        if gain is None:
+            gain = float(np.asarray(model.stats.get("gain", 1.0)))
+        self.mask *= np.float32(gain)
+        nx, ny = cfg.nx, cfg.ny
+        tdt = torch.float32 if dtype == _L.AMHD_F32 else torch.float64
+        self.x = torch.empty(n, nx, ny, 16, device="cuda", dtype=torch.bfloat16)
+        self.outs = FluxCNN.alloc_outputs(n, nx, ny)
+        self.theta = torch.ones(2 * n * nx * ny, device="cuda", dtype=torch.float32)   # theta, theta_mix
+        self.src = torch.zeros(n, 10, nx, ny, device="cuda", dtype=tdt)
+        self.work = torch.empty(32 * 64 * 4, device="cuda", dtype=torch.float64)
+        step = 10 * nx * ny * self.src.element_size()
+        self.src_ptrs = [self.src.data_ptr() + m * step for m in range(n)]
+        torch.cuda.synchronize()
+
+    def predict(self, pre):
+        cfg = self.cfg
+        x = features(list(pre), self.dtype, FEAT, cfg.nx, cfg.ny, self.model.stats, self.x, self.stream)
+        self.model.forward(x, stream=self.stream, bufs=self.outs)
+
+    def check(self, ptrs, dt) -> np.ndarray:
+        """(n, 4): min rho, min p, max|divB|, non-finite cells of U + dt S."""
+        cfg, n = self.cfg, len(ptrs)
+        out = np.empty((n, 4))
+        V = C.c_void_p * n
+        _L.check(_L.load().amhd_closure_check(V(*ptrs), V(*self.src_ptrs[:n]), _dvec(dt), self.dtype, n, cfg.nx,
+                                              cfg.ny, float(cfg.gamma), float(cfg.dx), float(cfg.dy),
+                                              _p(self.work), out.ctypes.data_as(C.POINTER(C.c_double)),
+                                              self.stream), "amhd_closure_check")
+        return out
+
+    def correct(self, pre, post, dt, div0):
+        """Source against the post-step states (bounds also from the pre-step
+        ones), guard, apply; returns (accepted mask, |divB| of the states now
+        current)."""
+        cfg, n = self.cfg, self.n
+        lib = _L.load()
+        V = C.c_void_p * n
+        lim = self.keep is not None
+        _L.check(lib.amhd_closure_source(_p(self.outs[-1]), V(*post), V(*self.src_ptrs), self.dtype, _int9(FEAT), n,
+                                         cfg.nx, cfg.ny, _f9(self.model.stats["f_scale"] * self.mask),
+                                         float(cfg.dx), float(cfg.dy), _dvec(dt), float(cfg.gamma),
+                                         float(self.keep) if lim else 0.0, _p(self.theta) if lim else None,
+                                         V(*pre) if lim else None, 0, self.stream), "amhd_closure_source")
+        r = self.check(post, dt)
+        ok = ((r[:, 0] > 0) & (r[:, 1] > 0) & (r[:, 3] == 0) & (r[:, 2] <= 1.5 * np.asarray(div0) + self.divb_floor)
+              & (np.asarray(dt) > 0))
+        idx = np.nonzero(ok)[0]
+        if len(idx):
+            k = len(idx)
+            W = C.c_void_p * k
+            _L.check(lib.amhd_closure_apply(W(*[post[i] for i in idx]), W(*[self.src_ptrs[i] for i in idx]),
+                                            _dvec([dt[i] for i in idx]), self.dtype, k, cfg.nx, cfg.ny, self.stream),
+                     "amhd_closure_apply")
+        div_now = r[:, 2].copy()
+        if not ok.all():
+            div_now[~ok] = self.check(post, np.zeros(n))[~ok, 2]
+        return ok, div_now
+
+
+def _dvec(v):
+    v = [float(x) for x in v]
+    return (C.c_double * len(v))(*v)
+
+
 class ClosureSolver:
-    """Coarse MHDSolver with the learned flux correction, its positivity
-    limiter and the plain-step fallback.  Every launch of a corrected step
-    (features, the four layers, limiter and source, the SSP-RK3 stages)
-    runs on the context's stream; the state is read in place."""
+    """Coarse MHDSolver with the learned flux correction, its limiters and
+    the plain-step fallback.  Every launch of a corrected step runs on the
+    context's stream and reads the state in place."""
 
     def __init__(self, cfg, model: FluxCNN | None, divb_floor: float = 1.0, keep: float | None = 0.5,
                  fields=None, gain: float | None = None):
@@ -514,43 +593,20 @@ class ClosureSolver:
         are applied, None for the model's own (all unless it was trained
         on fewer); the others keep the plain solver's fluxes.  gain: a
         factor on the corrections, None for the model's calibrated one
-        (stats["gain"], 1 if absent)."""
+        (stats["gain"], 1 if absent).  keep: the positivity limiter's bound
+        (None: no limiters)."""
         from solver import MHDSolver
 
         self.s = MHDSolver(cfg)
         self.model = model
-        self.divb_floor = divb_floor
-        self.keep = keep
-        if fields is None and model is not None:
-            fields = applied_fields(model.stats)
-        self.mask = np.ones(N_IN, dtype=np.float32)
-        if fields is not None:
-            self.mask[:] = 0.0
-            self.mask[list(fields)] = 1.0
-        if gain is None:
-            gain = float(np.asarray(model.stats.get("gain", 1.0))) if model is not None else 1.0
-        self.mask *= np.float32(gain)
         self.fallbacks = 0
-        self.limited = 0.0           # summed fraction of cells with theta < 1
-        self._bufs = None
+        self._opts = (keep, divb_floor, fields, gain)
+        self.cor = None
 
     def check(self):
         out = (C.c_double * 4)()
         _L.check(self.s.ctx.lib.amhd_state_check(self.s.ctx.h, out), "amhd_state_check")
         return tuple(out)
-
-    def _alloc(self):
-        torch = _L.require_gpu()
-        cfg, ctx = self.s.cfg, self.s.ctx
-        nx, ny = cfg.nx, cfg.ny
-        st = C.c_void_p()
-        _L.check(ctx.lib.amhd_get_stream(ctx.h, C.byref(st)), "amhd_get_stream")
-        self.stream = st
-        self.x = torch.empty(1, nx, ny, 16, device="cuda", dtype=torch.bfloat16)
-        self.outs = FluxCNN.alloc_outputs(1, nx, ny)
-        self.theta = torch.ones(2 * nx * ny, device="cuda", dtype=torch.float32)   # theta, theta_mix
-        self._bufs = True
-        torch.cuda.synchronize()
 
     def _state_ptr(self) -> int:
         ctx = self.s.ctx
@@ -558,75 +614,115 @@ class ClosureSolver:
         _L.check(ctx.lib.amhd_state_ptr(ctx.h, C.byref(state)), "amhd_state_ptr")
         return state.value
 
-    def predict(self):
-        """The network's face corrections for the current (pre-step) state,
-        left in self.outs[-1] (asynchronous on the context stream)."""
-        if self._bufs is None:
-            self._alloc()
-        cfg, ctx = self.s.cfg, self.s.ctx
-        self.pre = self._state_ptr()       # stays intact through the step (it is the rewind target)
-        x = features([self.pre], ctx.dtype, FEAT, cfg.nx, cfg.ny, self.model.stats, self.x, self.stream)
-        return self.model.forward(x, stream=self.stream, bufs=self.outs)
-
-    def correct(self, dt: float):
-        """U += dt * S of the predicted corrections on the current (post-step)
-        state, limited against that state; the concentration bounds also
-        take in the pre-step state (asynchronous)."""
-        cfg, ctx, st = self.s.cfg, self.s.ctx, self.model.stats
-        u = (C.c_void_p * 1)(self._state_ptr())
-        lim = self.keep is not None
-        _L.check(ctx.lib.amhd_closure_source(_p(self.outs[-1]), u, u, ctx.dtype, _int9(FEAT), 1, cfg.nx, cfg.ny,
-                                             _f9(st["f_scale"] * self.mask), float(cfg.dx), float(cfg.dy), float(dt),
-                                             float(cfg.gamma), float(self.keep) if lim else 0.0,
-                                             _p(self.theta) if lim else None,
-                                             (C.c_void_p * 1)(self.pre) if lim else None, 1, self.stream),
-                 "amhd_closure_source")
-
     def begin(self):
         """Upload an assigned state and take the guard's reference |divB|."""
         self.s._sync_state()
+        if self.model is not None and self.cor is None:
+            ctx = self.s.ctx
+            st = C.c_void_p()
+            _L.check(ctx.lib.amhd_get_stream(ctx.h, C.byref(st)), "amhd_get_stream")
+            self.cor = _Corrector(self.model, 1, self.s.cfg, ctx.dtype, st, *self._opts)
         self.div0 = self.check()[2]
 
-    def step(self, dt: float, on_plain=None, track_limiter: bool = False) -> bool:
-        """One coarse step of dt (cfg.glm_ch from a preceding compute_dt): the
-        plain SSP-RK3 step followed by U += dt * S(U_pre) -- the form the
-        network is trained on, r = (U_b - RK3(U_a)) / dt -- then the guard;
-        a failing step is rewound and taken again without the correction.
+    def step(self, dt: float | None = None, on_plain=None) -> bool:
+        """One coarse step -- the plain SSP-RK3 step followed by
+        U += dt * S(U_pre), the form the network is trained on,
+        r = (U_b - RK3(U_a)) / dt -- guarded: a failing correction is not
+        applied and the plain step stands.  dt None: the device loop's CFL
+        step (bounded by cfg.t_end); else that dt with the host-driven step.
         on_plain() runs between the plain step and the correction.  Returns
-        whether the correction was kept."""
+        whether the correction was applied, or None when no step ran."""
         s = self.s
-        s.dt = dt
-        if self.model is None:
+        pre = self._state_ptr() if self.model is not None else None
+        if self.model is not None:
+            self.cor.predict([pre])
+        if dt is None:
+            if s.advance(1, use_graph=False) == 0:   # one step: a 3-step graph would replay two no-ops
+                return None
+            dt = s.dt
+        else:
+            s.compute_dt()                 # sets cfg.glm_ch for the step
+            s.dt = dt
             s.step_ssprk3()
-            return True
-        self.predict()
-        s.step_ssprk3()
-        if on_plain is not None:
-            on_plain()
-        self.correct(dt)
-        rho_min, p_min, divb, nonfinite = self.check()
-        ok = rho_min > 0 and p_min > 0 and nonfinite == 0 and divb <= 1.5 * self.div0 + self.divb_floor
-        if track_limiter and self.keep is not None:
-            self.limited += float((self.theta[:cfg_cells(s)] < 1).float().mean())
-        if not ok:
-            _L.check(s.ctx.lib.amhd_rewind(s.ctx.h), "amhd_rewind")
-            s.step_ssprk3()
-            self.fallbacks += 1
-        self.div0 = self.check()[2]
-        return ok
-
-    def run(self, t_end: float, max_steps: int = 10 ** 7, track_limiter: bool = False):
-        """Coarse steps at the CFL time step until t_end."""
-        s = self.s
-        self.begin()
-        while s.t < t_end and s.step < max_steps:
-            dt = min(s.compute_dt(), t_end - s.t)
-            if dt <= 1e-16:
-                break
-            self.step(dt, track_limiter=track_limiter)
             s.t += dt
             s.step += 1
+        if self.model is None:
+            return True
+        post = self._state_ptr()
+        if on_plain is not None:
+            on_plain()
+        ok, div_now = self.cor.correct([pre], [post], [dt], [self.div0])
+        self.div0 = float(div_now[0])
+        if not ok[0]:
+            self.fallbacks += 1
+        return bool(ok[0])
+
+    def run(self, t_end: float, max_steps: int = 10 ** 7):
+        """Steps of the device loop (CFL dt) until t_end or max_steps."""
+        s = self.s
+        s.cfg.t_end = t_end
+        s.cfg.max_steps = max_steps
+        self.begin()
+        while s.t < t_end and s.step < max_steps:
+            if self.step() is None:
+                break
         return s
+
+
+class BatchClosure:
+    """The corrected coarse solver over n instances of one grid size: one
+    batched SSP-RK3 step (amhd_batch_run), one network pass over all
+    instances and one batched limited correction per step; each instance
+    keeps its own dt, t_end, guard and fallback.  An instance steps exactly
+    as ClosureSolver.run would step it alone."""
+
+    def __init__(self, cases, model: FluxCNN | None, divb_floor: float = 1.0, keep: float | None = 0.5,
+                 fields=None, gain: float | None = None):
+        torch = _L.require_gpu()
+        from amhd_batch import BatchSolver
+
+        cfgs = [c for c, _ in cases]
+        if len({(c.nx, c.ny) for c in cfgs}) != 1:
+            raise ValueError("BatchClosure: every instance must have the same grid")
+        self.b = BatchSolver(cases, precision=cfgs[0].precision)
+        self.model = model
+        self.n = len(cases)
+        self.fallbacks = np.zeros(self.n, dtype=np.int64)
+        dtype = _L.AMHD_F32 if self.b.precision in ("fp32", "float32") else _L.AMHD_F64
+        self.stream = _L.stream_handle(torch)
+        self.cor = (_Corrector(model, self.n, cfgs[0], dtype, self.stream, keep, divb_floor, fields, gain)
+                    if model is not None else None)
+
+    def _ptrs(self):
+        n = self.n
+        cur, prev = (C.c_void_p * n)(), (C.c_void_p * n)()
+        _L.check(self.b.lib.amhd_batch_state_ptrs(self.b.h, cur, prev), "amhd_batch_state_ptrs")
+        return [c for c in cur], [p for p in prev]
+
+    def run(self):
+        """Every instance to its own cfg.t_end / cfg.max_steps."""
+        torch = _L.require_gpu()
+        b = self.b
+        if self.cor is None:
+            while b.run(1).sum() > 0:
+                pass
+            return
+        cur, _ = self._ptrs()
+        div0 = self.cor.check(cur, np.zeros(self.n))[:, 2]
+        while True:
+            pre, _ = self._ptrs()
+            self.cor.predict(pre)
+            torch.cuda.current_stream().synchronize()
+            taken = b.run(1)
+            if taken.sum() == 0:
+                break
+            dts = (C.c_double * self.n)()
+            _L.check(b.lib.amhd_batch_last_dt(b.h, dts), "amhd_batch_last_dt")
+            dt = np.where(taken > 0, np.array(dts[:]), 0.0)
+            post, _ = self._ptrs()
+            ok, div0 = self.cor.correct(pre, post, dt, div0)
+            self.fallbacks += ((~ok) & (taken > 0)).astype(np.int64)
+            torch.cuda.current_stream().synchronize()
 
 
 def cfg_cells(s) -> int:
@@ -722,6 +818,52 @@ def fp64_fine_throughput(seed: int, steps: int = 2000, nx: int = 2048, ny: int =
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     return {"mcups": nx * ny * n / wall / 1e6, "wall_s_for_run": wall * steps / n, "steps": steps}
+
+
+def evaluate_batched(seeds, model, steps: int = 2000, nx: int = 2048, ny: int = 1024, log=None) -> dict:
+    """The batched form: the plain coarse solver and the closure each run
+    over all seeds at once (BatchSolver / BatchClosure, every instance to
+    its own fine run's end time), judged per seed as evaluate() does, with
+    the wall time of each batched run."""
+    torch = _L.require_gpu()
+    from amhd_diag import mixing_width
+
+    refs, cases = [], []
+    for seed in seeds:
+        fine = {}
+
+        def grab0(s):
+            fine["U0"] = restrict(s.ctx.download())
+
+        s = fine_run(seed, nx, ny, steps, grab0, [0])
+        Ufr = restrict(s.ctx.download())
+        ccfg = coarse_config(s.cfg)
+        ccfg.t_end, ccfg.max_steps = s.t, 10 ** 7
+        refs.append((Ufr, mixing_width(ghosted(Ufr).cpu().numpy(), ccfg)))
+        cases.append((ccfg, ghosted(fine["U0"]).cpu().numpy()))
+    out = {"seeds": list(seeds), "instances": len(cases)}
+    for tag, m in (("plain", None), ("closure", model)):
+        bc = BatchClosure(cases, m)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        bc.run()
+        torch.cuda.synchronize()
+        out[f"wall_{tag}_s"] = time.perf_counter() - t0
+        mw_err, l1 = [], []
+        for k, (Ufr, mw_fr) in enumerate(refs):
+            Uc = bc.b.download(k)
+            mw_err.append(abs(mixing_width(Uc, cases[k][0]) - mw_fr) / max(abs(mw_fr), 1e-30))
+            rho_c = Uc[0, 2:-2, 2:-2]
+            rho_f = Ufr[0].cpu().numpy()
+            l1.append(float(np.abs(rho_c - rho_f).mean() / np.abs(rho_f).mean()))
+        out[f"mw_err_{tag}_mean"] = float(np.mean(mw_err))
+        out[f"l1rho_{tag}_mean"] = float(np.mean(l1))
+        out[f"fallbacks_{tag}"] = int(bc.fallbacks.sum())
+        out[f"steps_{tag}"] = int(bc.b.steps.sum())
+    if log:
+        log("[closure] batched evaluation: " + ", ".join(f"{k}={v:.4g}" if isinstance(v, float) else f"{k}={v}"
+                                                      for k, v in out.items() if k != "seeds"))
+    return out
 
 
 def calibrate_gain(model, seeds, gains, steps: int = 2000, log=None):

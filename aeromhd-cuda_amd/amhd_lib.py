@@ -57,7 +57,6 @@ SIGNATURES = {
     "amhd_cfl_maxima": (_i, [_vp, _dp]),
     "amhd_dt_from_maxima": (_i, [_dp, _d, _d, _d, _dp, _dp]),
     "amhd_synchronize": (_i, [_vp]),
-    "amhd_rewind": (_i, [_vp]),
     "amhd_state_check": (_i, [_vp, _dp]),
     "amhd_state_ptr": (_i, [_vp, C.POINTER(_vp)]),
     "amhd_get_stream": (_i, [_vp, C.POINTER(_vp)]),
@@ -74,6 +73,8 @@ SIGNATURES = {
     "amhd_batch_info": (_i, [_vp, _ip, _ip, _ip]),
     "amhd_batch_upload": (_i, [_vp, _i, _vp, _i]),
     "amhd_batch_download": (_i, [_vp, _i, _vp, _i]),
+    "amhd_batch_state_ptrs": (_i, [_vp, C.POINTER(_vp), C.POINTER(_vp)]),
+    "amhd_batch_last_dt": (_i, [_vp, _dp]),
     "amhd_batch_run": (_i, [_vp, _i, _dp, _dp, _ip, _i, _dp, _ip]),
     "amhd_conv3x3_kpad": (_i, [_i]),
     "amhd_conv3x3_fwd": (_i, [_vp, _i, _i, _i, _i, _vp, _vp, _i, _i, _vp, _i, _vp]),
@@ -83,7 +84,9 @@ SIGNATURES = {
     "amhd_closure_features": (_i, [C.POINTER(_vp), _i, C.POINTER(_i), _i, _i, _i, C.POINTER(C.c_float),
                                    C.POINTER(C.c_float), _vp, _vp]),
     "amhd_closure_source": (_i, [_vp, C.POINTER(_vp), C.POINTER(_vp), _i, C.POINTER(_i), _i, _i, _i,
-                                 C.POINTER(C.c_float), _d, _d, _d, _d, _d, _vp, C.POINTER(_vp), _i, _vp]),
+                                 C.POINTER(C.c_float), _d, _d, _dp, _d, _d, _vp, C.POINTER(_vp), _i, _vp]),
+    "amhd_closure_check": (_i, [C.POINTER(_vp), C.POINTER(_vp), _dp, _i, _i, _i, _i, _d, _d, _d, _vp, _dp, _vp]),
+    "amhd_closure_apply": (_i, [C.POINTER(_vp), C.POINTER(_vp), _dp, _i, _i, _i, _i, _vp]),
 }
 
 _lib = None

@@ -659,15 +659,6 @@ int amhd_compute_rhs(void* ctx, double* U, double* R, double ch) {
     return 0;
 }
 
-int amhd_rewind(void* ctx) {
-    if (!ctx) return fail(-1, "amhd_rewind: null context");
-    Ctx* c = (Ctx*)ctx;
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    c->cur = (c->cur + 2) % 3;
-    c->part_valid = false;
-    return 0;
-}
-
 int amhd_state_check(void* ctx, double out[4]) {
     if (!ctx || !out) return fail(-1, "amhd_state_check: null argument");
     Ctx* c = (Ctx*)ctx;
@@ -1042,6 +1033,31 @@ int amhd_batch_download(void* handle, int k, double* U, int on_device) {
     const size_t gb = sizeof(double) * NV * (size_t)(b->nx[k] + 4) * (b->ny[k] + 4);
     HIP_TRY(hipMemcpyAsync(U, b->staging, gb, on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, b->stream));
     HIP_TRY(hipStreamSynchronize(b->stream));
+    return 0;
+}
+
+int amhd_batch_last_dt(void* handle, double* dt) {
+    if (!handle || !dt) return fail(-1, "amhd_batch_last_dt: null argument");
+    Batch* b = (Batch*)handle;
+    std::vector<StepCtl> h;
+    int rc = batch_read_ctl(b, h);
+    if (rc) return rc;
+    for (int k = 0; k < b->n; ++k) dt[k] = h[k].dt;   // a finished run leaves the last step's dt
+    return 0;
+}
+
+int amhd_batch_state_ptrs(void* handle, void** cur, void** prev) {
+    if (!handle || !cur) return fail(-1, "amhd_batch_state_ptrs: null argument");
+    Batch* b = (Batch*)handle;
+    std::vector<StepCtl> h;
+    int rc = batch_read_ctl(b, h);
+    if (rc) return rc;
+    for (int k = 0; k < b->n; ++k) {
+        cur[k] = b->inst[k].buf[h[k].cur];
+        if (prev) prev[k] = b->inst[k].buf[(h[k].cur + 2) % 3];
+        b->part_valid[k] = 0;   // the caller may edit the state: recompute the CFL partials
+        b->part_from_stage3[k] = 0;
+    }
     return 0;
 }
 

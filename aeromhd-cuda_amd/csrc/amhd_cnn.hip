@@ -22,6 +22,8 @@
 #include <cstdarg>
 #include <cstdio>
 #include <string>
+#include <vector>
+#include <cmath>
 
 #include "aeromhd.h"
 
@@ -280,7 +282,7 @@ bool ok_c(int c) { return c == 16 || c == 32; }
 // correction of the 9 x-face fluxes at i-1/2 (channels 0..8) and of the 9
 // y-face fluxes at j-1/2 (channels 9..17) in units of fscale.
 constexpr int NF = 9;
-constexpr int CHUNK = 64;     // instances per launch (pointers travel as kernel arguments)
+constexpr int CHUNK = 32;     // instances per launch (pointers and step sizes travel as kernel arguments)
 
 struct ConstPtrs { const void* p[CHUNK]; };
 struct MutPtrs { void* p[CHUNK]; };
@@ -289,6 +291,7 @@ struct Feat {
     float mean[NF], std[NF];
 };
 struct FluxScale { float s[NF]; int field[NF]; };
+struct StepDt { double dt[CHUNK], dtdx[CHUNK], dtdy[CHUNK]; };   // per instance: dt, dt/dx, dt/dy
 
 // x[(m, i, j)][f] = bf16((U_m[field[f]][i, j] - mean[f]) / std[f]), f < 9;
 // channels 9..15 zero.
@@ -342,8 +345,8 @@ __device__ __forceinline__ void load_faces(const float* __restrict__ dF, long p,
 // the limited correction keeps the bound in every cell.
 template <class real>
 __global__ __launch_bounds__(256) void closure_theta_kernel(const float* __restrict__ dF, ConstPtrs st, FluxScale fs,
-                                                            int n, int nx, int ny, double dtdx, double dtdy,
-                                                            double gamma, double keep, float* __restrict__ theta) {
+                                                            int n, int nx, int ny, StepDt sd, double gamma,
+                                                            double keep, float* __restrict__ theta) {
     const long plane = (long)nx * ny;
     const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= (long)n * plane) return;
@@ -352,6 +355,7 @@ __global__ __launch_bounds__(256) void closure_theta_kernel(const float* __restr
     const int i = (int)(c / ny), j = (int)(c - (long)i * ny);
     FaceCorr fc;
     load_faces(dF, k, i, j, nx, ny, fc);
+    const double dtdx = sd.dtdx[m], dtdy = sd.dtdy[m];
     const real* u = (const real*)st.p[m];
     double u0[10];
 #pragma unroll
@@ -458,7 +462,7 @@ __device__ __forceinline__ void limited_fluxes(const FaceCorr& fc, const FluxSca
 template <class real>
 __global__ __launch_bounds__(256) void closure_theta_mix_kernel(const float* __restrict__ dF, ConstPtrs st,
                                                                 ConstPtrs bst, bool has_b, FluxScale fs, int n,
-                                                                int nx, int ny, double dtdx, double dtdy,
+                                                                int nx, int ny, StepDt sd,
                                                                 const float* __restrict__ theta,
                                                                 float* __restrict__ theta_mix) {
     const long plane = (long)nx * ny;
@@ -470,6 +474,7 @@ __global__ __launch_bounds__(256) void closure_theta_mix_kernel(const float* __r
     FaceCorr fc;
     load_faces(dF, k, i, j, nx, ny, fc);
     const real* u = (const real*)st.p[m];
+    const double dtdx = sd.dtdx[m], dtdy = sd.dtdy[m];
     const FaceW w = face_weights(theta, k, i, j, nx, ny);
     const FaceW zero{0.f, 0.f, 0.f, 0.f};
     const Nbr q = neighbours(u, plane, c, i, j, nx, ny);
@@ -509,7 +514,7 @@ __global__ __launch_bounds__(256) void closure_source_kernel(const float* __rest
                                                              const float* __restrict__ theta,
                                                              const float* __restrict__ theta_mix, MutPtrs outs,
                                                              FluxScale fs, int n, int nx, int ny, double rdx,
-                                                             double rdy, double dt) {
+                                                             double rdy, StepDt sd) {
     const long plane = (long)nx * ny;
     const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= (long)n * plane) return;
@@ -527,7 +532,7 @@ __global__ __launch_bounds__(256) void closure_source_kernel(const float* __rest
     for (int f = 0; f < NF; ++f) {
         const double v = -((F.xr[f] - F.xl[f]) * rdx + (F.yr[f] - F.yl[f]) * rdy);
         real& o = s[fs.field[f] * plane + c];
-        if (ACC) o = real(double(o) + dt * v);
+        if (ACC) o = real(double(o) + sd.dt[m] * v);
         else o = real(v);
         named[fs.field[f]] = true;
     }
@@ -537,6 +542,76 @@ __global__ __launch_bounds__(256) void closure_source_kernel(const float* __rest
             if (!named[v]) s[v * plane + c] = real(0);
     }
 }
+
+// Guard quantities of U + dt S without writing it: per instance (blockIdx.y)
+// and workgroup, {min rho, min p, max |divB|, non-finite cells} with
+// p = (gamma-1)(E - |m|^2/(2 rho) - |B|^2/2) and divB as amhd_state_check
+// (central differences on rows and columns 1 .. n-2), all in FP64.
+template <class real>
+__global__ __launch_bounds__(256) void closure_check_kernel(ConstPtrs st, ConstPtrs src, StepDt sd, int nx, int ny,
+                                                            double gamma, double dx, double dy,
+                                                            double* __restrict__ part) {
+    const int m = blockIdx.y;
+    const real* u = (const real*)st.p[m];
+    const real* s = (const real*)src.p[m];
+    const double dt = sd.dt[m];
+    const long plane = (long)nx * ny;
+    auto val = [&](int v, long c) {
+        const double x = double(u[v * plane + c]);
+        return dt != 0.0 ? x + dt * double(s[v * plane + c]) : x;
+    };
+    double mr = 1e300, mp = 1e300, md = 0.0, nf = 0.0;
+    for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < plane; k += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(k / ny), j = (int)(k - (long)i * ny);
+        double w[10];
+        bool fin = true;
+#pragma unroll
+        for (int v = 0; v < 10; ++v) { w[v] = val(v, k); fin = fin && isfinite(w[v]); }
+        if (!fin) nf += 1.0;
+        const double ke = 0.5 * (w[1] * w[1] + w[2] * w[2] + w[3] * w[3]) / w[0];
+        const double me = 0.5 * (w[4] * w[4] + w[5] * w[5] + w[6] * w[6]);
+        mr = fmin(mr, w[0]);
+        mp = fmin(mp, (gamma - 1.0) * (w[7] - ke - me));
+        if (i >= 1 && i < nx - 1 && j >= 1 && j < ny - 1) {
+            const double d = (0.5 * (w[4] + val(4, k + ny)) - 0.5 * (val(4, k - ny) + w[4])) / dx +
+                             (0.5 * (w[5] + val(5, k + 1)) - 0.5 * (val(5, k - 1) + w[5])) / dy;
+            md = fmax(md, fabs(d));
+        }
+    }
+    __shared__ double sh[4][4];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int off = 32; off > 0; off >>= 1) {
+        mr = fmin(mr, __shfl_xor(mr, off, 64));
+        mp = fmin(mp, __shfl_xor(mp, off, 64));
+        md = fmax(md, __shfl_xor(md, off, 64));
+        nf += __shfl_xor(nf, off, 64);
+    }
+    if (lane == 0) { sh[0][wv] = mr; sh[1][wv] = mp; sh[2][wv] = md; sh[3][wv] = nf; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 1; q < (int)(blockDim.x >> 6); ++q) {
+            sh[0][0] = fmin(sh[0][0], sh[0][q]); sh[1][0] = fmin(sh[1][0], sh[1][q]);
+            sh[2][0] = fmax(sh[2][0], sh[2][q]); sh[3][0] += sh[3][q];
+        }
+        double* o = part + ((long)m * gridDim.x + blockIdx.x) * 4;
+        o[0] = sh[0][0]; o[1] = sh[1][0]; o[2] = sh[2][0]; o[3] = sh[3][0];
+    }
+}
+
+// U += dt S on all 10 planes of each instance.
+template <class real>
+__global__ __launch_bounds__(256) void closure_apply_kernel(MutPtrs st, ConstPtrs src, StepDt sd, int n, long plane) {
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (long)n * plane) return;
+    const int m = (int)(k / plane);
+    const long c = k - (long)m * plane;
+    real* u = (real*)st.p[m];
+    const real* s = (const real*)src.p[m];
+#pragma unroll
+    for (int v = 0; v < 10; ++v) u[v * plane + c] = real(double(u[v * plane + c]) + sd.dt[m] * double(s[v * plane + c]));
+}
+
+constexpr int CHECK_BLOCKS = 64;   // workgroups per instance in closure_check_kernel
 
 }  // namespace
 
@@ -660,14 +735,17 @@ int amhd_closure_features(const void* const* states, int dtype, const int* field
 }
 
 int amhd_closure_source(const float* dF, const void* const* states, void* const* srcs, int dtype, const int* fields,
-                        int n, int nx, int ny, const float* fscale, double dx, double dy, double dt, double gamma,
-                        double keep, float* theta, const void* const* bounds, int accumulate, void* stream) {
+                        int n, int nx, int ny, const float* fscale, double dx, double dy, const double* dt,
+                        double gamma, double keep, float* theta, const void* const* bounds, int accumulate,
+                        void* stream) {
     if (!dF || !states || !srcs || !fields || !fscale || n <= 0 || nx <= 0 || ny <= 0 || !(dx > 0) || !(dy > 0))
         return cfail(-1, "amhd_closure_source: bad argument");
     if (dtype != AMHD_F64 && dtype != AMHD_F32) return cfail(-1, "amhd_closure_source: bad dtype %d", dtype);
     if (fields[FRHO] != 0 || fields[FRC] != 9)
         return cfail(-1, "amhd_closure_source: fields[0] must be rho (0) and fields[8] rhoC (9)");
-    if ((theta || accumulate) && !(dt >= 0)) return cfail(-1, "amhd_closure_source: dt must be >= 0");
+    if ((theta || accumulate) && !dt) return cfail(-1, "amhd_closure_source: the limiter and accumulate need dt");
+    for (int m = 0; dt && m < n; ++m)
+        if (!(dt[m] >= 0)) return cfail(-1, "amhd_closure_source: dt[%d] must be >= 0", m);
     if (theta && !(keep >= 0 && keep < 1)) return cfail(-1, "amhd_closure_source: keep must be in [0, 1)");
     FluxScale fs;
     for (int f = 0; f < NF; ++f) {
@@ -689,6 +767,11 @@ int amhd_closure_source(const float* dF, const void* const* states, void* const*
             bs.p[m] = bounds ? bounds[m0 + m] : nullptr;
         }
         const bool hb = bounds != nullptr;
+        StepDt sd{};
+        for (int m = 0; m < cnt; ++m) {
+            const double t = dt ? dt[m0 + m] : 0.0;
+            sd.dt[m] = t; sd.dtdx[m] = t / dx; sd.dtdy[m] = t / dy;
+        }
         const long tot = (long)cnt * plane;
         const float* d = dF + (long)m0 * plane * 32;
         float* th = theta ? theta + (long)m0 * plane : nullptr;
@@ -696,21 +779,82 @@ int amhd_closure_source(const float* dF, const void* const* states, void* const*
         const unsigned g = (unsigned)((tot + 255) / 256);
         if (th) {
             if (dtype == AMHD_F64) {
-                closure_theta_kernel<double><<<g, 256, 0, s>>>(d, st, fs, cnt, nx, ny, dt / dx, dt / dy, gamma, keep, th);
-                closure_theta_mix_kernel<double><<<g, 256, 0, s>>>(d, st, bs, hb, fs, cnt, nx, ny, dt / dx, dt / dy, th,
-                                                                   tm);
+                closure_theta_kernel<double><<<g, 256, 0, s>>>(d, st, fs, cnt, nx, ny, sd, gamma, keep, th);
+                closure_theta_mix_kernel<double><<<g, 256, 0, s>>>(d, st, bs, hb, fs, cnt, nx, ny, sd, th, tm);
             } else {
-                closure_theta_kernel<float><<<g, 256, 0, s>>>(d, st, fs, cnt, nx, ny, dt / dx, dt / dy, gamma, keep, th);
-                closure_theta_mix_kernel<float><<<g, 256, 0, s>>>(d, st, bs, hb, fs, cnt, nx, ny, dt / dx, dt / dy, th,
-                                                                  tm);
+                closure_theta_kernel<float><<<g, 256, 0, s>>>(d, st, fs, cnt, nx, ny, sd, gamma, keep, th);
+                closure_theta_mix_kernel<float><<<g, 256, 0, s>>>(d, st, bs, hb, fs, cnt, nx, ny, sd, th, tm);
             }
             if (hipGetLastError() != hipSuccess) return cfail(-2, "amhd_closure_source: limiter launch failed");
         }
-#define SRC(R, A) closure_source_kernel<R, A><<<g, 256, 0, s>>>(d, st, th, tm, sp, fs, cnt, nx, ny, 1.0 / dx, 1.0 / dy, dt)
+#define SRC(R, A) closure_source_kernel<R, A><<<g, 256, 0, s>>>(d, st, th, tm, sp, fs, cnt, nx, ny, 1.0 / dx, 1.0 / dy, sd)
         if (dtype == AMHD_F64) { if (accumulate) SRC(double, true); else SRC(double, false); }
         else { if (accumulate) SRC(float, true); else SRC(float, false); }
 #undef SRC
         if (hipGetLastError() != hipSuccess) return cfail(-2, "amhd_closure_source: launch failed");
+    }
+    return 0;
+}
+
+int amhd_closure_check(const void* const* states, const void* const* srcs, const double* dt, int dtype, int n,
+                       int nx, int ny, double gamma, double dx, double dy, double* work, double* out, void* stream) {
+    if (!states || !srcs || !dt || !work || !out || n <= 0 || nx <= 0 || ny <= 0 || !(dx > 0) || !(dy > 0))
+        return cfail(-1, "amhd_closure_check: bad argument");
+    if (dtype != AMHD_F64 && dtype != AMHD_F32) return cfail(-1, "amhd_closure_check: bad dtype %d", dtype);
+    hipStream_t s = (hipStream_t)stream;
+    std::vector<double> h((size_t)CHUNK * CHECK_BLOCKS * 4);
+    for (int m0 = 0; m0 < n; m0 += CHUNK) {
+        const int cnt = n - m0 < CHUNK ? n - m0 : CHUNK;
+        ConstPtrs st{}, sr{};
+        StepDt sd{};
+        for (int m = 0; m < cnt; ++m) {
+            if (!states[m0 + m] || !srcs[m0 + m]) return cfail(-1, "amhd_closure_check: null pointer %d", m0 + m);
+            st.p[m] = states[m0 + m];
+            sr.p[m] = srcs[m0 + m];
+            sd.dt[m] = dt[m0 + m];
+        }
+        const dim3 grid(CHECK_BLOCKS, cnt);
+        if (dtype == AMHD_F64) closure_check_kernel<double><<<grid, 256, 0, s>>>(st, sr, sd, nx, ny, gamma, dx, dy, work);
+        else closure_check_kernel<float><<<grid, 256, 0, s>>>(st, sr, sd, nx, ny, gamma, dx, dy, work);
+        if (hipGetLastError() != hipSuccess) return cfail(-2, "amhd_closure_check: launch failed");
+        const size_t bytes = sizeof(double) * (size_t)cnt * CHECK_BLOCKS * 4;
+        if (hipMemcpyAsync(h.data(), work, bytes, hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)
+            return cfail(-2, "amhd_closure_check: copy failed");
+        for (int m = 0; m < cnt; ++m) {
+            double* o = out + 4 * (size_t)(m0 + m);
+            const double* p = h.data() + (size_t)m * CHECK_BLOCKS * 4;
+            o[0] = p[0]; o[1] = p[1]; o[2] = p[2]; o[3] = p[3];
+            for (int b = 1; b < CHECK_BLOCKS; ++b) {
+                o[0] = std::fmin(o[0], p[4 * b]); o[1] = std::fmin(o[1], p[4 * b + 1]);
+                o[2] = std::fmax(o[2], p[4 * b + 2]); o[3] += p[4 * b + 3];
+            }
+        }
+    }
+    return 0;
+}
+
+int amhd_closure_apply(void* const* states, const void* const* srcs, const double* dt, int dtype, int n, int nx,
+                       int ny, void* stream) {
+    if (!states || !srcs || !dt || n < 0 || nx <= 0 || ny <= 0) return cfail(-1, "amhd_closure_apply: bad argument");
+    if (dtype != AMHD_F64 && dtype != AMHD_F32) return cfail(-1, "amhd_closure_apply: bad dtype %d", dtype);
+    const long plane = (long)nx * ny;
+    hipStream_t s = (hipStream_t)stream;
+    for (int m0 = 0; m0 < n; m0 += CHUNK) {
+        const int cnt = n - m0 < CHUNK ? n - m0 : CHUNK;
+        MutPtrs st{};
+        ConstPtrs sr{};
+        StepDt sd{};
+        for (int m = 0; m < cnt; ++m) {
+            if (!states[m0 + m] || !srcs[m0 + m]) return cfail(-1, "amhd_closure_apply: null pointer %d", m0 + m);
+            st.p[m] = states[m0 + m];
+            sr.p[m] = srcs[m0 + m];
+            sd.dt[m] = dt[m0 + m];
+        }
+        const unsigned g = (unsigned)(((long)cnt * plane + 255) / 256);
+        if (dtype == AMHD_F64) closure_apply_kernel<double><<<g, 256, 0, s>>>(st, sr, sd, cnt, plane);
+        else closure_apply_kernel<float><<<g, 256, 0, s>>>(st, sr, sd, cnt, plane);
+        if (hipGetLastError() != hipSuccess) return cfail(-2, "amhd_closure_apply: launch failed");
     }
     return 0;
 }

@@ -44,6 +44,11 @@ def final_evaluation(rep, model, eval_seeds, variants, args):
     fp64 = rep["fp64_fine"]["mcups"]
     for t in tags:
         summ[f"speedup_{t}_vs_fp64_fine"] = fine_cells / summ[f"wall_{t}_s_mean"] / 1e6 / fp64
+    bt = cl.evaluate_batched(eval_seeds, model, steps=args.fine_steps, log=log)
+    fine_cells = 2048 * 1024 * args.fine_steps * len(eval_seeds)
+    for t in ("plain", "closure"):
+        bt[f"speedup_{t}_vs_fp64_fine"] = fine_cells / bt[f"wall_{t}_s"] / 1e6 / fp64
+    rep["batched"] = bt
     rep["summary"] = summ
     log(f"[closure] summary {summ}")
     json.dump(rep, open(args.report, "w"), indent=1, default=float)

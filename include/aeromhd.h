@@ -139,8 +139,6 @@ int amhd_global_sums(void* ctx, double out[4]);
 int amhd_concentration_profile(void* ctx, double* mean, double* var);
 
 /* Learned-closure hooks (no counterpart in the reference).
- * amhd_rewind: after a host-driven step, make the previous state current
- * again (the step is discarded).
  * amhd_state_check: out = {min rho, min p, max|divB|, non-finite cells}.
  * amhd_state_ptr: device address of the current interior-layout state
  * (10 planes of nx*ny cells, the context's precision; valid until the next
@@ -148,7 +146,6 @@ int amhd_concentration_profile(void* ctx, double* mean, double* var);
  * (the cached CFL partials are dropped).
  * amhd_get_stream: the context's stream (for work that must be ordered
  * with its steps without a host synchronisation). */
-int amhd_rewind(void* ctx);
 int amhd_state_check(void* ctx, double out[4]);
 int amhd_state_ptr(void* ctx, void** state);
 int amhd_get_stream(void* ctx, void** stream);
@@ -203,6 +200,14 @@ int amhd_batch_download(void* batch, int k, double* U, int on_device);
  * max_steps[k], counted in step_inout[k]); t_inout, step_inout: [n]. */
 int amhd_batch_run(void* batch, int n_steps, const double* cfl, const double* t_end, const int* max_steps,
                    int use_graph, double* t_inout, int* step_inout);
+/* Device addresses of every instance's current state (cur[k]) and of the
+ * state before its last step (prev[k], may be NULL), interior layout, valid
+ * until the next amhd_batch_run (synchronises).  The caller may edit the
+ * current states before the next run (the CFL partials are recomputed). */
+int amhd_batch_state_ptrs(void* batch, void** cur, void** prev);
+/* dt of every instance's last step of the last amhd_batch_run (host array
+ * of n; meaningful for the instances that took a step; synchronises). */
+int amhd_batch_last_dt(void* batch, double* dt);
 
 /* ---- learned flux closure (no counterpart in the reference): 3x3
  * convolutions on BF16 matrix cores (v_mfma_f32_16x16x32_bf16) with FP32
@@ -244,6 +249,8 @@ int amhd_mfma_probe(const float* A, const float* B, float* D, void* stream);
  * accumulate = 0: srcs[m] (10 planes, dtype) = S, other planes 0;
  * accumulate = 1: srcs[m] += dt * S on the named planes (srcs[m] may be
  * states[m]: the post-step correction of a state in place).
+ * dt: host array of n step sizes (needed by the limiter and by accumulate;
+ * may be NULL otherwise).
  * theta: NULL (w = wm = 1) or 2*n*nx*ny floats of workspace for the face
  * weights w = min over the face's two cells of theta_c, wm likewise of
  * theta_mix_c.  theta_c = min over the cell's faces of the largest t in
@@ -258,8 +265,19 @@ int amhd_mfma_probe(const float* A, const float* B, float* D, void* stream);
 int amhd_closure_features(const void* const* states, int dtype, const int* fields, int n, int nx, int ny,
                           const float* mean, const float* std, void* x, void* stream);
 int amhd_closure_source(const float* dF, const void* const* states, void* const* srcs, int dtype, const int* fields,
-                        int n, int nx, int ny, const float* fscale, double dx, double dy, double dt, double gamma,
-                        double keep, float* theta, const void* const* bounds, int accumulate, void* stream);
+                        int n, int nx, int ny, const float* fscale, double dx, double dy, const double* dt,
+                        double gamma, double keep, float* theta, const void* const* bounds, int accumulate,
+                        void* stream);
+/* The closure's guard without writing the state: for instance m, out[4m..4m+3]
+ * = {min rho, min p, max|divB|, non-finite cells} of states[m] + dt[m] * srcs[m]
+ * (10 planes each; p = (gamma-1)(E - |m|^2/(2 rho) - |B|^2/2), divB as
+ * amhd_state_check), FP64; work: device workspace of 32*64*4 doubles;
+ * host out; synchronises.  dt[m] = 0 checks the state itself.
+ * amhd_closure_apply: states[m] += dt[m] * srcs[m] on all 10 planes. */
+int amhd_closure_check(const void* const* states, const void* const* srcs, const double* dt, int dtype, int n,
+                       int nx, int ny, double gamma, double dx, double dy, double* work, double* out, void* stream);
+int amhd_closure_apply(void* const* states, const void* const* srcs, const double* dt, int dtype, int n, int nx,
+                       int ny, void* stream);
 
 #ifdef __cplusplus
 }

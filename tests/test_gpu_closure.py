@@ -154,12 +154,28 @@ def test_closure_glue_vs_oracle(torch_gpu, dtype):
 
     def run(th, out=sp, acc=0, bnd=None):
         L.check(lib.amhd_closure_source(_p(dF), V(*ptrs), V(*out), dtype, cl._int9(cl.FEAT), n, nx, ny, cl._f9(fs),
-                                        dx, dy, dt, gamma, keep, _p(th) if th is not None else None, bnd, acc,
-                                        L.stream_handle(torch)), "amhd_closure_source")
+                                        dx, dy, cl._dvec([dt] * n), gamma, keep, _p(th) if th is not None else None,
+                                        bnd, acc, L.stream_handle(torch)), "amhd_closure_source")
         torch.cuda.synchronize()
         return src.cpu().numpy().astype(np.float64)
 
     S = run(theta)
+    # the guard quantities of U + dt S (nothing written), FP64 against NumPy
+    out = np.empty((n, 4))
+    work = torch.empty(32 * 64 * 4, device="cuda", dtype=torch.float64)
+    L.check(lib.amhd_closure_check(V(*ptrs), V(*sp), cl._dvec([dt] * n), dtype, n, nx, ny, gamma, dx, dy, _p(work),
+                                   out.ctypes.data_as(C.POINTER(C.c_double)), L.stream_handle(torch)),
+            "amhd_closure_check")
+    Wc = Uh + dt * S
+    pc = (gamma - 1) * (Wc[:, 7] - 0.5 * (Wc[:, 1] ** 2 + Wc[:, 2] ** 2 + Wc[:, 3] ** 2) / Wc[:, 0]
+                        - 0.5 * (Wc[:, 4] ** 2 + Wc[:, 5] ** 2 + Wc[:, 6] ** 2))
+    bx, by = Wc[:, 4], Wc[:, 5]
+    div = ((0.5 * (bx[:, 1:-1, 1:-1] + bx[:, 2:, 1:-1]) - 0.5 * (bx[:, :-2, 1:-1] + bx[:, 1:-1, 1:-1])) / dx
+           + (0.5 * (by[:, 1:-1, 1:-1] + by[:, 1:-1, 2:]) - 0.5 * (by[:, 1:-1, :-2] + by[:, 1:-1, 1:-1])) / dy)
+    np.testing.assert_allclose(out[:, 0], Wc[:, 0].min(axis=(1, 2)), rtol=1e-12)
+    np.testing.assert_allclose(out[:, 1], pc.min(axis=(1, 2)), rtol=1e-12)
+    np.testing.assert_allclose(out[:, 2], np.abs(div).max(axis=(1, 2)), rtol=1e-12)
+    assert np.all(out[:, 3] == 0)
     dFh = dF.cpu().numpy().astype(np.float64)
     fs64 = fs.astype(np.float64)
     Ch = co.concentration(Uh)
@@ -235,10 +251,10 @@ def _coarse_sieve(seed):
     return cfg, cons_from_prim_host(W, cfg.gamma)
 
 
-def test_closure_fallback_rewinds_exactly(torch_gpu):
-    """A guard that rejects every corrected step (divB floor -1e30) makes
-    every step a rewind plus a plain step: the run then equals the plain
-    coarse solver's bit for bit, step for step."""
+def test_closure_fallback_is_the_plain_step(torch_gpu):
+    """A guard that rejects every correction (divB floor -1e30) leaves every
+    step the plain one: the run equals the plain coarse solver's bit for
+    bit, step for step."""
     import amhd_closure as cl
 
     model = cl.FluxCNN.load(os.path.join(PKG, "closure_weights.npz"))
@@ -269,3 +285,30 @@ def test_closure_guard_catches_bad_corrections(torch_gpu):
     print(f"fallbacks {cs.fallbacks}/{cs.s.step}, rho_min {rho_min:.3e}, p_min {p_min:.3e}", flush=True)
     assert cs.fallbacks > 0
     assert nonfinite == 0 and rho_min > 0 and p_min > 0
+
+
+def test_batched_closure_equals_single_instances(torch_gpu):
+    """Three held-out seeds in one BatchClosure (one batched step, one
+    network pass, one batched correction per step) end bit for bit where
+    ClosureSolver takes each alone, with the same step counts."""
+    import amhd_closure as cl
+
+    model = cl.FluxCNN.load(os.path.join(PKG, "closure_weights.npz"))
+    cases = []
+    for seed, t_end in ((213, 0.004), (214, 0.0035), (215, 0.003)):
+        cfg, U = _coarse_sieve(seed)
+        cfg.t_end = t_end
+        cases.append((cfg, U))
+    singles = []
+    for cfg, U in cases:
+        cs = cl.ClosureSolver(cfg, model)
+        cs.s.U = U.copy()
+        cs.run(cfg.t_end)
+        singles.append((cs.s.ctx.download_numpy(), cs.s.step, cs.s.t))
+    bc = cl.BatchClosure(cases, model)
+    bc.run()
+    for k, (U1, steps, t) in enumerate(singles):
+        print(f"instance {k}: steps {bc.b.steps[k]} / {steps}, fallbacks {bc.fallbacks[k]}", flush=True)
+        assert bc.b.steps[k] == steps
+        assert bc.b.t[k] == t
+        np.testing.assert_array_equal(bc.b.download(k), U1)
