@@ -725,10 +725,6 @@ class BatchClosure:
             torch.cuda.current_stream().synchronize()
 
 
-def cfg_cells(s) -> int:
-    return s.cfg.nx * s.cfg.ny
-
-
 # ------------------------------------------------------- evaluation
 VARIANTS = {
     "closure": dict(keep=0.5),                       # the model's own fields and gain

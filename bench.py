@@ -311,6 +311,9 @@ def main():
         copy_calibration()
     log(f"[bench] workload {args.workload}: warmup {args.warmup}, steps {args.steps}")
     m = gpu_measure(args.workload, args.steps, args.warmup, args.seed, args.rows or None)
+    if not m["finite"]:
+        print(f"bench.py: the {args.workload} state is not finite after the timed steps", file=sys.stderr)
+        sys.exit(3)
     log(f"[bench] {m['mcups']:.1f} Mcell/s, stage us {m['stage_avg_us']}, achieved {m['achieved_gbs']:.0f} GB/s")
 
     secondary = []
