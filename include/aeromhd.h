@@ -259,9 +259,9 @@ int amhd_mfma_probe(const float* A, const float* B, float* D, void* stream);
  * within the min/max of C over the cell and its face neighbours, taken in
  * states and, if bounds (n device addresses, or NULL) is given, in those
  * states as well (the states before the step, for a post-step
- * correction: Zalesak's bounds).  By
- * convexity every cell of states + dt*S keeps both bounds, and S stays a
- * flux difference, so it conserves exactly. */
+ * correction: Zalesak's bounds).  By convexity every cell of states + dt*S
+ * keeps both bounds, and S stays a flux difference, so it conserves
+ * exactly. */
 int amhd_closure_features(const void* const* states, int dtype, const int* fields, int n, int nx, int ny,
                           const float* mean, const float* std, void* x, void* stream);
 int amhd_closure_source(const float* dF, const void* const* states, void* const* srcs, int dtype, const int* fields,
