@@ -117,14 +117,15 @@ def test_conversions_muscl_bc_bounds_dt_bitwise(torch_gpu, orc):
 
 
 @pytest.mark.parametrize("powell", [False, True])
-@pytest.mark.parametrize("bcx", ["periodic", "extrapolation"])
-def test_compute_rhs_bitwise(torch_gpu, orc, powell, bcx):
+@pytest.mark.parametrize("bcx,bcy", [("periodic", "periodic"), ("extrapolation", "periodic"),
+                                     ("extrapolation", "outflow"), ("periodic", "outflow")])
+def test_compute_rhs_bitwise(torch_gpu, orc, powell, bcx, bcy):
     from config import Config
     from solver import MHDSolver
 
-    say("compute_rhs", powell, bcx)
+    say("compute_rhs", powell, bcx, bcy)
     cfg = Config(nx=70, ny=130, x_min=0, x_max=1, y_min=0, y_max=130 / 70, gamma=5 / 3, bc_x_type=bcx,
-                 bc_y_type="periodic", powell_source=powell, B_transverse=0.3 if powell else 0.0)
+                 bc_y_type=bcy, powell_source=powell, B_transverse=0.3 if powell else 0.0)
     U = prim_to_cons_np(smooth_state(70, 130, seed=5, amp=0.3), cfg.gamma)
     U[:, 30:40, :] *= 1.5        # a jump so that limiters and every HLLD region engage
     s = MHDSolver(cfg)
@@ -156,6 +157,30 @@ def test_single_step_bitwise_with_tilings(torch_gpu, orc):
         s.step_ssprk3()
         np.testing.assert_array_equal(s.U.cpu().numpy(), Uo)
         assert s._cumulative_boundary_energy == pytest.approx(bfo * dt, rel=1e-12, abs=1e-300)
+
+
+def test_single_step_bitwise_y_outflow(torch_gpu, orc):
+    """Zero-gradient y boundaries (solver.py:178-197, no psi zeroing in y):
+    one step bit for bit against the oracle with the y strips cut in two
+    tilings, and the jump placed across the y boundary."""
+    from config import Config
+    from solver import MHDSolver
+
+    cfg = Config(nx=64, ny=125, x_min=0, x_max=1, y_min=0, y_max=125 / 64, gamma=5 / 3,
+                 bc_x_type="extrapolation", bc_y_type="outflow", B_transverse=0.25)
+    U = prim_to_cons_np(smooth_state(64, 125, seed=13, amp=0.3), cfg.gamma)
+    U[:, :, 2:6] *= 1.4
+    U[:, :, -6:-2] *= 0.7
+    dt, ch = orc.compute_dt(orc.cons_to_prim(U, cfg.gamma), cfg)
+    Uo, _ = orc.step(U, cfg, dt, ch)
+    for rows in (5, 64):
+        say("y outflow step rows", rows)
+        s = MHDSolver(cfg)
+        s.ctx.set_rows(rows)
+        s.U = U
+        s.dt, cfg.glm_ch = dt, ch
+        s.step_ssprk3()
+        np.testing.assert_array_equal(s.U.cpu().numpy(), Uo)
 
 
 def test_brio_wu_100_steps_fp64(torch_gpu, orc):
