@@ -233,10 +233,12 @@ __device__ __forceinline__ void stage_wave(const StageParams<real>& a, const int
     };
 
     // prologue: W at rows i0, i0+1, the x-differences and the flux through
-    // face i0-1/2.  The row loop keeps three primitive rows (A, B, C) and
-    // three x-flux rows (F0, F1, F2) whose roles rotate with period 3, so
-    // the window slides without register copies.
+    // face i0-1/2.  The row loop keeps three primitive rows (A, B, C), their
+    // conserved values (QA, QB, QC: the stage input of the row the combine
+    // needs, loaded once) and three x-flux rows (F0, F1, F2), whose roles
+    // rotate with period 3, so the window slides without register copies.
     real A[NV], B[NV], C[NV];
+    real QA[NV], QB[NV], QC[NV];
     real F0[NV], F1[NV], F2[NV];
     real xl[NV], dxp[NV];   // left state of face i+1/2; W_{i+1} - W_i
     real bxm;               // Bx of row i-1 (Powell)
@@ -244,8 +246,10 @@ __device__ __forceinline__ void stage_wave(const StageParams<real>& a, const int
         real wm2[NV], wm1[NV], fa[NV], fb[NV];
         prim_row(i0 - 2, wm2);
         prim_row(i0 - 1, wm1);
-        prim_row(i0, A);
-        prim_row(i0 + 1, B);
+        load_in(i0, QA);
+        cons_to_prim<real, P>(QA, A, kc.gamma, psi_ghost(i0));
+        load_in(i0 + 1, QB);
+        cons_to_prim<real, P>(QB, B, kc.gamma, psi_ghost(i0 + 1));
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             const real dmm = wm1[v] - wm2[v], dm0 = A[v] - wm1[v];
@@ -267,16 +271,18 @@ __device__ __forceinline__ void stage_wave(const StageParams<real>& a, const int
     load_in(i0 + 2, upre);
 
     // One output row i.  w0 = W_i, wp1 = W_{i+1}; wp2 receives W_{i+2};
-    // fxl = F_{i-1/2}; fxr receives F_{i+1/2}.
-    auto row = [&](int i, real (&w0)[NV], real (&wp1)[NV], real (&wp2)[NV], const real (&fxl)[NV],
-                   real (&fxr)[NV]) __attribute__((always_inline)) {
+    // ucur = U_i (stage input); qp2 receives U_{i+2}; fxl = F_{i-1/2}; fxr
+    // receives F_{i+1/2}.
+    auto row = [&](int i, real (&w0)[NV], real (&wp1)[NV], real (&wp2)[NV], const real (&ucur)[NV],
+                   real (&qp2)[NV], const real (&fxl)[NV], real (&fxr)[NV]) __attribute__((always_inline)) {
         cons_to_prim<real, P>(upre, wp2, kc.gamma, psi_ghost(i + 2));
+#pragma unroll
+        for (int v = 0; v < NV; ++v) qp2[v] = upre[v];
         if (i + 1 < i1) load_in(i + 3, upre);
 
-        // stage input and base at this row (halo lanes read a valid
-        // remapped column and never store)
-        real ucur[NV], ubase[NV];
-        load_row(r_us, i, ucur);
+        // base at this row (halo lanes read a valid remapped column and
+        // never store)
+        real ubase[NV];
         if constexpr (STAGE >= 2) load_row(r_u0, i, ubase);
 
         // ---- x face i+1/2: slope at row i+1 from the differences
@@ -382,11 +388,11 @@ __device__ __forceinline__ void stage_wave(const StageParams<real>& a, const int
 
     for (int i = i0;;) {
         if (i >= i1) break;
-        row(i, A, B, C, F0, F1);
+        row(i, A, B, C, QA, QC, F0, F1);
         if (++i >= i1) break;
-        row(i, B, C, A, F1, F2);
+        row(i, B, C, A, QB, QA, F1, F2);
         if (++i >= i1) break;
-        row(i, C, A, B, F2, F0);
+        row(i, C, A, B, QC, QB, F2, F0);
         ++i;
     }
 
