@@ -183,6 +183,55 @@ def test_single_step_bitwise_y_outflow(torch_gpu, orc):
         np.testing.assert_array_equal(s.U.cpu().numpy(), Uo)
 
 
+@pytest.mark.parametrize("nx,ny,bcx,bcy", [(2, 2, "periodic", "periodic"), (2, 61, "extrapolation", "outflow"),
+                                           (3, 60, "periodic", "periodic"), (5, 121, "extrapolation", "periodic"),
+                                           (130, 2, "periodic", "outflow"), (7, 119, "extrapolation", "outflow")])
+def test_small_and_ragged_grids(torch_gpu, orc, nx, ny, bcx, bcy):
+    """The smallest grids the ABI accepts (2 cells on an axis, where the
+    ghosts of both sides copy the same interior cells) and y extents one
+    cell either side of the 60-cell strip width: one step and 20 steps of
+    the host loop bit for bit, the step in two row tilings.  The ghosts are
+    filled first, as the reference's initialize leaves them: the oracle's
+    CFL scan reads them, the library keeps only the interior."""
+    from config import Config
+    from solver import MHDSolver, to_numpy
+
+    cfg = Config(nx=nx, ny=ny, x_min=0, x_max=1, y_min=0, y_max=ny / nx, gamma=5 / 3, bc_x_type=bcx,
+                 bc_y_type=bcy, B_transverse=0.25, t_end=1e9)
+    U = orc.apply_bc(prim_to_cons_np(smooth_state(nx, ny, seed=nx * 1000 + ny, amp=0.3), cfg.gamma), cfg)
+    dt, ch = orc.compute_dt(orc.cons_to_prim(U, cfg.gamma), cfg)
+    Uo, _ = orc.step(U, cfg, dt, ch)
+    for rows in (1, nx):
+        say(f"small grid {nx}x{ny} {bcx}/{bcy} rows {rows}")
+        s = MHDSolver(cfg)
+        s.ctx.set_rows(rows)
+        s.U = U
+        s.dt, cfg.glm_ch = dt, ch
+        s.step_ssprk3()
+        np.testing.assert_array_equal(s.U.cpu().numpy(), Uo)
+
+    Uo, to, so, _, _ = orc.run_loop(U, cfg, 20)
+    s = MHDSolver(cfg)
+    s.U = U
+    s.t, s.step = 0.0, 0
+    while s.step < 20:
+        s.dt = s.compute_dt()
+        s.step_ssprk3()
+        s.t += s.dt
+        s.step += 1
+    assert so == 20 and s.t == to
+    np.testing.assert_array_equal(to_numpy(s.U), Uo)
+
+
+def test_grid_below_two_cells_is_rejected(torch_gpu):
+    from config import Config
+    from solver import MHDSolver
+
+    for nx, ny in ((1, 8), (8, 1)):
+        with pytest.raises(RuntimeError, match="too small"):
+            MHDSolver(Config(nx=nx, ny=ny, x_min=0, x_max=1, y_min=0, y_max=1))
+
+
 def test_brio_wu_100_steps_fp64(torch_gpu, orc):
     """Brio-Wu 800 x 4, gamma 2, CFL 0.4, 100 steps: GPU FP64 vs oracle."""
     from amhd_ics import brio_wu, cons_from_prim_host
