@@ -137,6 +137,42 @@ def test_compute_rhs_bitwise(torch_gpu, orc, powell, bcx, bcy):
     np.testing.assert_array_equal(R, Ro)
 
 
+def test_compute_rhs_at_any_base_address(torch_gpu, orc):
+    """The stage kernel builds its buffer descriptors from the 64-bit base
+    addresses of its arrays.  U and R placed inside one allocation of just
+    over 2 GiB, once at its start and once at the next 2^31 boundary (so bit
+    31 of the address's low word differs between the two), give the oracle's
+    result bit for bit in both places."""
+    import torch
+    from config import Config
+    from solver import MHDSolver
+
+    cfg = Config(nx=70, ny=130, x_min=0, x_max=1, y_min=0, y_max=130 / 70, gamma=5 / 3,
+                 bc_x_type="extrapolation", bc_y_type="periodic", powell_source=True, B_transverse=0.3)
+    U = prim_to_cons_np(smooth_state(70, 130, seed=6, amp=0.3), cfg.gamma)
+    U[:, 30:40, :] *= 1.5
+    Ro, Uo = orc.compute_rhs(U, cfg, 3.7)
+    nb = U.size * 8
+    big = torch.empty(2 ** 31 + 2 * nb + 4096, dtype=torch.uint8, device="cuda")
+    base = big.data_ptr()
+    s = MHDSolver(cfg)
+    bits = []
+    for off in (0, (((base >> 31) + 1) << 31) - base):
+        addr = base + off
+        bits.append((addr >> 31) & 1)
+        say(f"compute_rhs at address {addr:#x}")
+        Ut = big[off:off + nb].view(torch.float64).view(U.shape)
+        Rt = big[off + nb:off + 2 * nb].view(torch.float64).view(U.shape)
+        Ut.copy_(torch.from_numpy(U))
+        torch.cuda.synchronize()
+        rc = s.ctx.lib.amhd_compute_rhs(s.ctx.h, C.c_void_p(Ut.data_ptr()), C.c_void_p(Rt.data_ptr()), 3.7)
+        assert rc == 0, s.ctx.lib.amhd_last_error()
+        np.testing.assert_array_equal(Ut.cpu().numpy(), Uo)
+        np.testing.assert_array_equal(Rt.cpu().numpy(), Ro)
+    assert sorted(bits) == [0, 1]
+    del big
+
+
 # ------------------------------------------------------------ fused steps
 def test_single_step_bitwise_with_tilings(torch_gpu, orc):
     from config import Config
