@@ -142,7 +142,7 @@ def batch_measure(n: int, steps: int, warmup: int, seed: int, seq_steps: int = 2
         s.ctx.close()
     return {"workload": f"batched sweep: {n} single-mode RMI instances, grids 256x128..1024x512 (seed {seed}), "
                         f"FP32, one persistent launch per RK stage, hipGraph replay",
-            "value": round(cells * steps / wall / 1e6, 2), "unit": "Mcell-updates/s", "dtype": "fp32",
+            "value": round(cells * steps / wall / 1e6, 2), "unit": "Mcell-updates/s", "dtype": "f32",
             "steps": steps, "ms_per_step": round(wall * 1e3 / steps, 4), "cells": cells,
             "tasks": info["tasks"], "rows_per_task": rows or info["rows"],
             "sequential_value": round(cells * seq_steps / seq_wall / 1e6, 2)}
@@ -324,7 +324,7 @@ def main():
             log(f"[bench] secondary {wl}")
             x = gpu_measure(wl, 200, 20, args.seed, None)
             secondary.append({"workload": x["desc"], "value": round(x["mcups"], 2), "unit": "Mcell-updates/s",
-                              "dtype": "fp32" if x["esize"] == 4 else "f64",
+                              "dtype": "f32" if x["esize"] == 4 else "f64",
                               "ms_per_step": round(x["ms_per_step"], 4),
                               "achieved_GBps": round(x["achieved_gbs"], 1),
                               "frac": round(x["achieved_gbs"] / HBM_PEAK_GBS, 4),
@@ -363,7 +363,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32" if m["esize"] == 4 else "f64",
+        "dtype": "f32" if m["esize"] == 4 else "f64",
         "data": "synthetic (seeded initial condition generated on the host)",
         "config": {"workload": m["desc"], "nx": m["cfg"].nx, "ny": m["cfg"].ny, "rows_per_segment": m["rows"],
                    "cfl": m["cfg"].cfl, "powell": bool(m["cfg"].powell_source and m["cfg"].B_transverse > 0)},
