@@ -819,7 +819,7 @@ def fp64_fine_throughput(seed: int, steps: int = 2000, nx: int = 2048, ny: int =
 def evaluate_batched(seeds, model, steps: int = 2000, nx: int = 2048, ny: int = 1024, log=None) -> dict:
     """The batched form: the plain coarse solver and the closure each run
     over all seeds at once (BatchSolver / BatchClosure, every instance to
-    its own fine run's end time), judged per seed as evaluate() does, with
+    its own fine run's end time), scored per seed as evaluate() does, with
     the wall time of each batched run."""
     torch = _L.require_gpu()
     from amhd_diag import mixing_width

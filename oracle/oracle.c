@@ -1,8 +1,8 @@
 /*
  * oracle.c -- CPU restatement of the reference's 2-D ideal-MHD update.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP
- * product path and the CPU baseline timed beside it.  Only tests/,
+ * TEST INFRASTRUCTURE ONLY.  The tests compare the HIP product path with
+ * this library, and it is the CPU baseline timed beside it.  Only tests/,
  * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.
  * Nothing in the product (aeromhd-cuda_amd/) links or calls it.
  *
@@ -25,7 +25,7 @@
  *   orc_powell_source       solver.py:242-266
  *   orc_compute_rhs         solver.py:268-308
  *   orc_step_ssprk3         solver.py:310-333
- *   orc_run                 solver.py:475-482 / main.py:66-73 driver loop
+ *   orc_run                 solver.py:475-482 / main.py:66-73 time loop
  *
  * Pinning (what checks each function independently of this text) is listed
  * in the header of oracle/oracle.py and in the tests under tests/.
@@ -609,7 +609,7 @@ double orc_step_ssprk3(double* U, const orc_params* p, double dt, double ch, dou
     return bflux;
 }
 
-/* Driver loop of solver.py:475-482 (diagnostics excluded).  Runs until
+/* Time loop of solver.py:475-482 (diagnostics excluded).  Runs until
  * t >= t_end or steps >= max_steps (or dt <= 1e-16).  The psi decay uses
  * libm exp here; the product evaluates it the same way on the host side
  * (numpy) for the step-by-step API. */

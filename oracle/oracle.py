@@ -3,8 +3,9 @@
 ctypes wrapper over liboracle.so, the plain-C restatement in oracle.c of
 physics.py:43-379 and solver.py:158-333 on the reference's ghosted layout
 (float64, -ffp-contract=off).  Only tests/, __graft_entry__.smoke() and
-bench.py's cpu_baseline leg use it, and only as the checker / the timed CPU
-baseline: the product (aeromhd-cuda_amd/) never imports it.
+bench.py's cpu_baseline leg use it, and only as what results are compared
+with / the timed CPU baseline: the product (aeromhd-cuda_amd/) never
+imports it.
 
 What pins each oracle function independently of its own text (tests under
 tests/ named in brackets):
@@ -208,7 +209,7 @@ def step(U, cfg, dt, ch, decay=None):
 
 
 def run_loop(U, cfg, n_steps, t=0.0):
-    """The driver loop of main.py:66-73 / solver.py:475-482 in Python over
+    """The time loop of main.py:66-73 / solver.py:475-482 in Python over
     the oracle (host numpy exp for the decay, as the reference).  Returns
     (U, t, steps, last_dt, cumulative boundary energy)."""
     U = _f64(U).copy()

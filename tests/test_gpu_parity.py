@@ -23,7 +23,7 @@ def say(*a):
 
 
 def gpu_loop(cfg, W, n_steps, use_advance=False):
-    """The reference driver loop (main.py:66-73) over the drop-in solver."""
+    """The reference time loop (main.py:66-73) over the drop-in solver."""
     from physics import cons_to_prim, prim_to_cons
     from solver import MHDSolver, to_numpy
 
