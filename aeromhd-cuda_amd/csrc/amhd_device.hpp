@@ -208,18 +208,23 @@ __host__ __device__ inline FluxConst<real> make_flux_const(double gamma, double 
 // where ch = 1.5 max(cf + |v|, 1) >= 1.5).  Then SL <= -ch < 0 < ch <= SR,
 // every face lies in the star fan, and f[4], f[8] are the GLM values
 // (physics.py:265-268); the supersonic branches are compiled out.
-template <class real, class P, bool GLM = false>
+//
+// FLOORED = true: the caller has already floored rho and p of both states
+// with the same comparison (physics.py:329-336 floors the MUSCL face
+// states), so the floors of physics.py:63-72 would return their inputs
+// unchanged (NaN included) and are skipped.
+template <class real, class P, bool GLM = false, bool FLOORED = false>
 __device__ __forceinline__ real hlld_face(const real (&wl)[NV], const real (&wr)[NV], real (&f)[NV],
                                           const FluxConst<real>& k) {
     const real half = real(0.5);
-    real rhoL = wl[0] < k.floor_rho ? k.floor_rho : wl[0];
-    real rhoR = wr[0] < k.floor_rho ? k.floor_rho : wr[0];
+    real rhoL = (!FLOORED && wl[0] < k.floor_rho) ? k.floor_rho : wl[0];
+    real rhoR = (!FLOORED && wr[0] < k.floor_rho) ? k.floor_rho : wr[0];
     real vxL = wl[1], vyL = wl[2], vzL = wl[3];
     real vxR = wr[1], vyR = wr[2], vzR = wr[3];
     real bxL = wl[4], byL = wl[5], bzL = wl[6];
     real bxR = wr[4], byR = wr[5], bzR = wr[6];
-    real pL = wl[7] < k.floor_pr ? k.floor_pr : wl[7];
-    real pR = wr[7] < k.floor_pr ? k.floor_pr : wr[7];
+    real pL = (!FLOORED && wl[7] < k.floor_pr) ? k.floor_pr : wl[7];
+    real pR = (!FLOORED && wr[7] < k.floor_pr) ? k.floor_pr : wr[7];
     real psiL = wl[8], psiR = wr[8];
     real cL = wl[9], cR = wr[9];
     const bool glm = GLM || k.ch > real(0);
@@ -240,18 +245,33 @@ __device__ __forceinline__ real hlld_face(const real (&wl)[NV], const real (&wr)
     const real ptR = pR + half * bmR2;
 
     // fast magnetosonic speeds (physics.py:90-102)
-    real irL, irR;
-    P::rcp2(rhoL, rhoR, irL, irR);
-    real aL2 = P::divi(k.gamma * pL, rhoL, irL);
-    real aR2 = P::divi(k.gamma * pR, rhoR, irR);
-    real tmpL = aL2 + P::divi(bmL2, rhoL, irL);
-    real tmpR = aR2 + P::divi(bmR2, rhoR, irR);
-    real discL = tmpL * tmpL - P::divi(real(4) * aL2 * bx2, rhoL, irL);
-    real discR = tmpR * tmpR - P::divi(real(4) * aR2 * bx2, rhoR, irR);
-    if (discL < real(0)) discL = real(0);
-    if (discR < real(0)) discR = real(0);
-    const real cfL = P::sqrt_(fmax(half * (tmpL + P::sqrt_(discL)), real(0)));
-    const real cfR = P::sqrt_(fmax(half * (tmpR + P::sqrt_(discR)), real(0)));
+    real cfL, cfR;
+    if constexpr (P::exact) {
+        real irL, irR;
+        P::rcp2(rhoL, rhoR, irL, irR);
+        real aL2 = P::divi(k.gamma * pL, rhoL, irL);
+        real aR2 = P::divi(k.gamma * pR, rhoR, irR);
+        real tmpL = aL2 + P::divi(bmL2, rhoL, irL);
+        real tmpR = aR2 + P::divi(bmR2, rhoR, irR);
+        real discL = tmpL * tmpL - P::divi(real(4) * aL2 * bx2, rhoL, irL);
+        real discR = tmpR * tmpR - P::divi(real(4) * aR2 * bx2, rhoR, irR);
+        if (discL < real(0)) discL = real(0);
+        if (discR < real(0)) discR = real(0);
+        cfL = P::sqrt_(fmax(half * (tmpL + P::sqrt_(discL)), real(0)));
+        cfR = P::sqrt_(fmax(half * (tmpR + P::sqrt_(discR)), real(0)));
+    } else {
+        // the same speed with the 1/rho factored out: with A = gamma p,
+        // B = |B|^2, X = Bx^2, cf^2 = (A + B + sqrt((A + B)^2 - 4 A X)) / (2 rho);
+        // A + B + sqrt(..) >= 0, so the outer clamp is not needed
+        real irL, irR;
+        P::rcp2(rhoL, rhoR, irL, irR);
+        const real aL = k.gamma * pL, aR = k.gamma * pR;
+        const real sL = aL + bmL2, sR = aR + bmR2;
+        const real dL = fmax(sL * sL - real(4) * aL * bx2, real(0));
+        const real dR = fmax(sR * sR - real(4) * aR * bx2, real(0));
+        cfL = P::sqrt_((sL + P::sqrt_(dL)) * (half * irL));
+        cfR = P::sqrt_((sR + P::sqrt_(dR)) * (half * irR));
+    }
 
     real sl = fmin(vxL - cfL, vxR - cfR);
     real sr = fmax(vxL + cfL, vxR + cfR);
@@ -261,7 +281,13 @@ __device__ __forceinline__ real hlld_face(const real (&wl)[NV], const real (&wr)
     // contact speed (physics.py:111-120)
     const real dsL = sl - vxL, dsR = sr - vxR;
     real den = dsR * rhoR - dsL * rhoL;
-    if (fabs(den) < P::eps_den()) den = (den >= real(0)) ? P::eps_den() : -P::eps_den();
+    if constexpr (P::exact) {
+        if (fabs(den) < P::eps_den()) den = (den >= real(0)) ? P::eps_den() : -P::eps_den();
+    } else {
+        // dsR >= cfR >= 0 >= dsL (SR >= vxR + cfR and SL <= vxL - cfL as
+        // rounded), so den >= 0 and the clamp above is max(den, eps)
+        den = fmax(den, P::eps_den());
+    }
     real sm = P::div(dsR * rhoR * vxR - dsL * rhoL * vxL - ptR + ptL, den);
     {
         const real e = P::eps_sm(sl, sr);
@@ -271,8 +297,15 @@ __device__ __forceinline__ real hlld_face(const real (&wl)[NV], const real (&wr)
     const real pts = ptL + rhoL * dsL * (sm - vxL);
 
     // star states (physics.py:122-154)
-    real dslsm = sl - sm; if (fabs(dslsm) < P::eps_den()) dslsm = P::eps_den();
-    real dsrsm = sr - sm; if (fabs(dsrsm) < P::eps_den()) dsrsm = P::eps_den();
+    real dslsm = sl - sm, dsrsm = sr - sm;
+    if constexpr (P::exact || !GLM) {
+        if (fabs(dslsm) < P::eps_den()) dslsm = P::eps_den();
+        if (fabs(dsrsm) < P::eps_den()) dsrsm = P::eps_den();
+    }
+    // (fast path with GLM: SL <= -ch < 0 < ch <= SR gives SR - SL >= |SL|, |SR|,
+    // so the SM margin 2^-20 (SR - SL) exceeds an ulp of SL and SR: SM differs
+    // from both by at least one ulp, |SL - SM|, |SR - SM| >= 2^-24 ch >> 1e-14,
+    // and the clamps cannot fire)
     real idslsm, idsrsm;
     P::rcp2(dslsm, dsrsm, idslsm, idsrsm);
     const real rhoLS = fmax(P::divi(rhoL * dsL, dslsm, idslsm), k.floor_rho);
@@ -289,14 +322,25 @@ __device__ __forceinline__ real hlld_face(const real (&wl)[NV], const real (&wr)
     real idnL, idnR;
     P::rcp2(dnL, dnR, idnL, idnR);
 
-    const real vyLS = vyL - P::divi(bxf * byL * (sm - vxL), dnL, idnL);
-    const real vzLS = vzL - P::divi(bxf * bzL * (sm - vxL), dnL, idnL);
-    const real byLS = P::divi(byL * (rhoL * dsL * dsL - bx2), dnL, idnL);
-    const real bzLS = P::divi(bzL * (rhoL * dsL * dsL - bx2), dnL, idnL);
-    const real vyRS = vyR - P::divi(bxf * byR * (sm - vxR), dnR, idnR);
-    const real vzRS = vzR - P::divi(bxf * bzR * (sm - vxR), dnR, idnR);
-    const real byRS = P::divi(byR * (rhoR * dsR * dsR - bx2), dnR, idnR);
-    const real bzRS = P::divi(bzR * (rhoR * dsR * dsR - bx2), dnR, idnR);
+    real vyLS, vzLS, byLS, bzLS, vyRS, vzRS, byRS, bzRS;
+    if constexpr (P::exact) {
+        vyLS = vyL - P::divi(bxf * byL * (sm - vxL), dnL, idnL);
+        vzLS = vzL - P::divi(bxf * bzL * (sm - vxL), dnL, idnL);
+        byLS = P::divi(byL * (rhoL * dsL * dsL - bx2), dnL, idnL);
+        bzLS = P::divi(bzL * (rhoL * dsL * dsL - bx2), dnL, idnL);
+        vyRS = vyR - P::divi(bxf * byR * (sm - vxR), dnR, idnR);
+        vzRS = vzR - P::divi(bxf * bzR * (sm - vxR), dnR, idnR);
+        byRS = P::divi(byR * (rhoR * dsR * dsR - bx2), dnR, idnR);
+        bzRS = P::divi(bzR * (rhoR * dsR * dsR - bx2), dnR, idnR);
+    } else {
+        // the factors shared by the two transverse components, once per side
+        const real tL = bxf * (sm - vxL) * idnL, tR = bxf * (sm - vxR) * idnR;
+        const real gL = (rhoL * dsL * dsL - bx2) * idnL, gR = (rhoR * dsR * dsR - bx2) * idnR;
+        vyLS = vyL - byL * tL; vzLS = vzL - bzL * tL;
+        byLS = byL * gL; bzLS = bzL * gL;
+        vyRS = vyR - byR * tR; vzRS = vzR - bzR * tR;
+        byRS = byR * gR; bzRS = bzR * gR;
+    }
 
     // region: supersonic L / R, or the star fan split by the contact.  Only
     // the side K that holds the face needs its total energy, star energy,
@@ -379,18 +423,26 @@ __device__ __forceinline__ real hlld_face(const real (&wl)[NV], const real (&wr)
     const real us[NV] = {rhos, rhos * sm, rhos * vys, rhos * vzs, bxf, bys, bzs, es, psif, rhos * c};
     const real uds[NV] = {rhos, rhos * sm, rhos * vyDS, rhos * vzDS, bxf, byDS, bzDS, eds, psif, rhos * c};
 
+    if constexpr (P::exact) {
 #pragma unroll
-    for (int v = 0; v < NV; ++v) {
-        if (GLM && (v == 4 || v == 8)) continue;   // set below
-        real fs = fk[v] + sk * (us[v] - uk[v]);
-        real fds;
-        if constexpr (P::exact) {
-            fds = fs + sak * (uds[v] - us[v]);
-        } else {
-            // components 0, 1, 4, 8, 9 of U** equal those of U*
-            fds = (v == 0 || v == 1 || v == 4 || v == 8 || v == 9) ? fs : fs + sak * (uds[v] - us[v]);
+        for (int v = 0; v < NV; ++v) {
+            if (GLM && (v == 4 || v == 8)) continue;   // set below
+            const real fs = fk[v] + sk * (us[v] - uk[v]);
+            const real fds = fs + sak * (uds[v] - us[v]);
+            f[v] = star ? (dbl ? fds : fs) : fk[v];
         }
-        f[v] = star ? (dbl ? fds : fs) : fk[v];
+    } else {
+        // F* = F_K + S_K (U*_K - U_K); F** = F* + S*_K (U**_K - U*_K), the
+        // second term taken with S*_K = 0 outside the double-star region;
+        // components 0, 1, 4, 8, 9 of U** equal those of U*
+        const real sks = star ? sk : real(0);
+        const real sas = dbl ? sak : real(0);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            if (GLM && (v == 4 || v == 8)) continue;   // set below
+            const real fs = fk[v] + sks * (us[v] - uk[v]);
+            f[v] = (v == 0 || v == 1 || v == 4 || v == 8 || v == 9) ? fs : fs + sas * (uds[v] - us[v]);
+        }
     }
     if (glm) {  // physics.py:265-268 (every glm face is in the star fan)
         f[4] = psif;
@@ -400,13 +452,13 @@ __device__ __forceinline__ real hlld_face(const real (&wl)[NV], const real (&wr)
 }
 
 // y-face flux: rotate (vx,vy),(Bx,By), solve, rotate back (physics.py:373-379)
-template <class real, class P, bool GLM = false>
+template <class real, class P, bool GLM = false, bool FLOORED = false>
 __device__ __forceinline__ real hlld_face_y(const real (&wl)[NV], const real (&wr)[NV], real (&f)[NV],
                                             const FluxConst<real>& k) {
     real a[NV] = {wl[0], wl[2], wl[1], wl[3], wl[5], wl[4], wl[6], wl[7], wl[8], wl[9]};
     real b[NV] = {wr[0], wr[2], wr[1], wr[3], wr[5], wr[4], wr[6], wr[7], wr[8], wr[9]};
     real g[NV];
-    real s = hlld_face<real, P, GLM>(a, b, g, k);
+    real s = hlld_face<real, P, GLM, FLOORED>(a, b, g, k);
     f[0] = g[0]; f[1] = g[2]; f[2] = g[1]; f[3] = g[3]; f[4] = g[5];
     f[5] = g[4]; f[6] = g[6]; f[7] = g[7]; f[8] = g[8]; f[9] = g[9];
     return s;

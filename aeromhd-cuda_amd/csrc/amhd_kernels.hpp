@@ -185,6 +185,9 @@ __device__ __forceinline__ void stage_wave(const StageParams<real>& a, const int
     const real dtr = real(dt_d);
     const real dxr = real(a.dx), dyr = real(a.dy);
     const real idx = real(1.0 / a.dx), idy = real(1.0 / a.dy);
+    // fast path: weight of dt L(U) in the stage's combine, times 1/dx, 1/dy, 1
+    constexpr double wst = STAGE == 1 ? 1.0 : (STAGE == 2 ? 0.25 : 2.0 / 3.0);
+    const real cx = real(wst * dt_d / a.dx), cy = real(wst * dt_d / a.dy), cs = real(wst * dt_d);
 
     constexpr unsigned ES = sizeof(real);
     const unsigned pb = (unsigned)a.plane * ES;             // plane stride, bytes
@@ -261,7 +264,7 @@ __device__ __forceinline__ void stage_wave(const StageParams<real>& a, const int
             xl[v] = A[v] + h_0;
         }
         floor_face(fa); floor_face(fb); floor_face(xl);
-        hlld_face<real, P, GLM>(fa, fb, F0, kc);
+        hlld_face<real, P, GLM, true>(fa, fb, F0, kc);
         bxm = wm1[iBX];
     }
 
@@ -297,7 +300,7 @@ __device__ __forceinline__ void stage_wave(const StageParams<real>& a, const int
                 xn[v] = wp1[v] + h1;
             }
             floor_face(xr);
-            hlld_face<real, P, GLM>(xl, xr, fxr, kc);
+            hlld_face<real, P, GLM, true>(xl, xr, fxr, kc);
 #pragma unroll
             for (int v = 0; v < NV; ++v) xl[v] = xn[v];
             floor_face(xl);
@@ -319,7 +322,7 @@ __device__ __forceinline__ void stage_wave(const StageParams<real>& a, const int
                 yr[v] = from_next(w0[v] - h);
             }
             floor_face(yl); floor_face(yr);
-            hlld_face_y<real, P, GLM>(yl, yr, fy, kc);
+            hlld_face_y<real, P, GLM, true>(yl, yr, fy, kc);
         }
 
         // ---- RHS: ((0 - dFx/dx) - dFy/dy) + S   (solver.py:297-306)
@@ -332,27 +335,55 @@ __device__ __forceinline__ void stage_wave(const StageParams<real>& a, const int
         }
         bxm = w0[iBX];
         real unew[NV];
+        if constexpr (P::exact || STAGE == 0) {
 #pragma unroll
-        for (int v = 0; v < NV; ++v) {
-            const real fyl = from_prev(fy[v]);
-            real r = real(0) - P::divi(fxr[v] - fxl[v], dxr, idx);
-            r = r - P::divi(fy[v] - fyl, dyr, idy);
-            if constexpr (POWELL) {
-                // S = -divB (0, B, v, v.B, 0, 0)   (solver.py:256-264)
-                real s = real(0);
-                if (v == MX) s = -divb * w0[iBX];
-                if (v == MY) s = -divb * w0[iBY];
-                if (v == MZ) s = -divb * w0[iBZ];
-                if (v == BX) s = -divb * w0[iVX];
-                if (v == BY) s = -divb * w0[iVY];
-                if (v == BZ) s = -divb * w0[iVZ];
-                if (v == EN) s = -divb * (w0[iVX] * w0[iBX] + w0[iVY] * w0[iBY] + w0[iVZ] * w0[iBZ]);
-                r = r + s;
+            for (int v = 0; v < NV; ++v) {
+                const real fyl = from_prev(fy[v]);
+                real r = real(0) - P::divi(fxr[v] - fxl[v], dxr, idx);
+                r = r - P::divi(fy[v] - fyl, dyr, idy);
+                if constexpr (POWELL) {
+                    // S = -divB (0, B, v, v.B, 0, 0)   (solver.py:256-264)
+                    real s = real(0);
+                    if (v == MX) s = -divb * w0[iBX];
+                    if (v == MY) s = -divb * w0[iBY];
+                    if (v == MZ) s = -divb * w0[iBZ];
+                    if (v == BX) s = -divb * w0[iVX];
+                    if (v == BY) s = -divb * w0[iVY];
+                    if (v == BZ) s = -divb * w0[iVZ];
+                    if (v == EN) s = -divb * (w0[iVX] * w0[iBX] + w0[iVY] * w0[iBY] + w0[iVZ] * w0[iBZ]);
+                    r = r + s;
+                }
+                if constexpr (STAGE == 0) unew[v] = r;
+                else if constexpr (STAGE == 1) unew[v] = ucur[v] + dtr * r;
+                else if constexpr (STAGE == 2) unew[v] = real(0.75) * ubase[v] + real(0.25) * (ucur[v] + dtr * r);
+                else unew[v] = real(1.0 / 3.0) * ubase[v] + real(2.0 / 3.0) * (ucur[v] + dtr * r);
             }
-            if constexpr (STAGE == 0) unew[v] = r;
-            else if constexpr (STAGE == 1) unew[v] = ucur[v] + dtr * r;
-            else if constexpr (STAGE == 2) unew[v] = real(0.75) * ubase[v] + real(0.25) * (ucur[v] + dtr * r);
-            else unew[v] = real(1.0 / 3.0) * ubase[v] + real(2.0 / 3.0) * (ucur[v] + dtr * r);
+        } else {
+            // The same combine with the stage's weight on dt L(U) folded
+            // into per-launch constants: U1 = U + dt L, U2 = 3/4 U0 + 1/4 U1
+            // + (dt/4) L, U3 = 1/3 U0 + 2/3 U2 + (2 dt/3) L, L = -dFx/dx -
+            // dFy/dy + S, each flux difference entering with one FMA.
+            const real pw = divb * cs;   // the Powell source enters as -divB (..) * c dt
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                const real fyl = from_prev(fy[v]);
+                real acc;
+                if constexpr (STAGE == 1) acc = ucur[v];
+                else if constexpr (STAGE == 2) acc = real(0.75) * ubase[v] + real(0.25) * ucur[v];
+                else acc = real(1.0 / 3.0) * ubase[v] + real(2.0 / 3.0) * ucur[v];
+                acc = acc + (fxl[v] - fxr[v]) * cx;
+                acc = acc + (fyl - fy[v]) * cy;
+                if constexpr (POWELL) {
+                    if (v == MX) acc = acc - pw * w0[iBX];
+                    if (v == MY) acc = acc - pw * w0[iBY];
+                    if (v == MZ) acc = acc - pw * w0[iBZ];
+                    if (v == BX) acc = acc - pw * w0[iVX];
+                    if (v == BY) acc = acc - pw * w0[iVY];
+                    if (v == BZ) acc = acc - pw * w0[iVZ];
+                    if (v == EN) acc = acc - pw * (w0[iVX] * w0[iBX] + w0[iVY] * w0[iBY] + w0[iVZ] * w0[iBZ]);
+                }
+                unew[v] = acc;
+            }
         }
         if constexpr (STAGE >= 1) {
             // enforce_scalar_bounds (solver.py:199-204)
