@@ -124,7 +124,9 @@ __device__ __forceinline__ int remap(int i, int n, int periodic) {
 // boundary, where solver.py:174-176 sets psi = 0.
 template <class real, class P>
 __device__ __forceinline__ void cons_to_prim(const real (&u)[NV], real (&w)[NV], real gamma, bool psi_zero) {
-    real rho = u[RHO] > real(FLOOR_RHO) ? u[RHO] : real(FLOOR_RHO);
+    real rho;
+    if constexpr (P::exact) rho = u[RHO] > real(FLOOR_RHO) ? u[RHO] : real(FLOOR_RHO);
+    else rho = fmax(u[RHO], real(FLOOR_RHO));
     const real ri = P::rcp(rho);
     real vx = u[MX] * ri, vy = u[MY] * ri, vz = u[MZ] * ri;
     real bx = u[BX], by = u[BY], bz = u[BZ];
@@ -132,10 +134,16 @@ __device__ __forceinline__ void cons_to_prim(const real (&u)[NV], real (&w)[NV],
     w[iBX] = bx; w[iBY] = by; w[iBZ] = bz;
     w[iPSI] = psi_zero ? real(0) : u[PSI];
     w[iCLR] = u[RHOC] * ri;
-    real ke = real(0.5) * rho * (vx * vx + vy * vy + vz * vz);
-    real me = real(0.5) * (bx * bx + by * by + bz * bz);
-    real p = (gamma - real(1)) * (u[EN] - ke - me);
-    w[iPR] = p > real(FLOOR_PR) ? p : real(FLOOR_PR);
+    if constexpr (P::exact) {
+        real ke = real(0.5) * rho * (vx * vx + vy * vy + vz * vz);
+        real me = real(0.5) * (bx * bx + by * by + bz * bz);
+        real p = (gamma - real(1)) * (u[EN] - ke - me);
+        w[iPR] = p > real(FLOOR_PR) ? p : real(FLOOR_PR);
+    } else {
+        // rho v^2 = m.v (v = m / rho); p = (gamma - 1) (E - (m.v + B^2) / 2)
+        const real s = u[MX] * vx + u[MY] * vy + u[MZ] * vz + bx * bx + by * by + bz * bz;
+        w[iPR] = fmax((gamma - real(1)) * (u[EN] - real(0.5) * s), real(FLOOR_PR));
+    }
 }
 
 // van Leer limited slope, physics.py:325-328, from the one-sided

@@ -99,10 +99,18 @@ __device__ __forceinline__ int xcd_logical_block() {
     return b < nb8 ? (b & 7) * (nb8 >> 3) + (b >> 3) : b;
 }
 
-template <class real>
+// Floors of the MUSCL face states (physics.py:329-336).  The exact policy
+// keeps the reference's comparison (a NaN passes through); the fast one
+// uses max, one instruction per value.
+template <class real, class P>
 __device__ __forceinline__ void floor_face(real (&w)[NV]) {
-    if (w[iRHO] < real(FLOOR_RHO)) w[iRHO] = real(FLOOR_RHO);
-    if (w[iPR] < real(FLOOR_PR)) w[iPR] = real(FLOOR_PR);
+    if constexpr (P::exact) {
+        if (w[iRHO] < real(FLOOR_RHO)) w[iRHO] = real(FLOOR_RHO);
+        if (w[iPR] < real(FLOOR_PR)) w[iPR] = real(FLOOR_PR);
+    } else {
+        w[iRHO] = fmax(w[iRHO], real(FLOOR_RHO));
+        w[iPR] = fmax(w[iPR], real(FLOOR_PR));
+    }
 }
 
 template <class real>
@@ -187,7 +195,8 @@ __device__ __forceinline__ void stage_wave(const StageParams<real>& a, const int
     const real idx = real(1.0 / a.dx), idy = real(1.0 / a.dy);
     // fast path: weight of dt L(U) in the stage's combine, times 1/dx, 1/dy, 1
     constexpr double wst = STAGE == 1 ? 1.0 : (STAGE == 2 ? 0.25 : 2.0 / 3.0);
-    const real cx = real(wst * dt_d / a.dx), cy = real(wst * dt_d / a.dy), cs = real(wst * dt_d);
+    const real cx = real(wst * dt_d / a.dx), cy = real(wst * dt_d / a.dy);
+    const real pcx = real(0.5 * wst * dt_d / a.dx), pcy = real(0.5 * wst * dt_d / a.dy);
 
     constexpr unsigned ES = sizeof(real);
     const unsigned pb = (unsigned)a.plane * ES;             // plane stride, bytes
@@ -263,7 +272,7 @@ __device__ __forceinline__ void stage_wave(const StageParams<real>& a, const int
             fb[v] = A[v] - h_0;
             xl[v] = A[v] + h_0;
         }
-        floor_face(fa); floor_face(fb); floor_face(xl);
+        floor_face<real, P>(fa); floor_face<real, P>(fb); floor_face<real, P>(xl);
         hlld_face<real, P, GLM, true>(fa, fb, F0, kc);
         bxm = wm1[iBX];
     }
@@ -299,11 +308,11 @@ __device__ __forceinline__ void stage_wave(const StageParams<real>& a, const int
                 xr[v] = wp1[v] - h1;
                 xn[v] = wp1[v] + h1;
             }
-            floor_face(xr);
+            floor_face<real, P>(xr);
             hlld_face<real, P, GLM, true>(xl, xr, fxr, kc);
 #pragma unroll
             for (int v = 0; v < NV; ++v) xl[v] = xn[v];
-            floor_face(xl);
+            floor_face<real, P>(xl);
         }
 
         // ---- y face j+1/2 of row i (neighbours through DPP lane shifts)
@@ -321,18 +330,23 @@ __device__ __forceinline__ void stage_wave(const StageParams<real>& a, const int
                 yl[v] = w0[v] + h;
                 yr[v] = from_next(w0[v] - h);
             }
-            floor_face(yl); floor_face(yr);
+            floor_face<real, P>(yl); floor_face<real, P>(yr);
             hlld_face_y<real, P, GLM, true>(yl, yr, fy, kc);
         }
 
         // ---- RHS: ((0 - dFx/dx) - dFy/dy) + S   (solver.py:297-306)
         real divb = real(0);
-        if constexpr (POWELL) {
+        if constexpr (POWELL && (P::exact || STAGE == 0)) {
             // solver.py:249-254, on the BC-filled primitive field
             const real half = real(0.5);
             divb = P::divi(half * (w0[iBX] + wp1[iBX]) - half * (bxm + w0[iBX]), dxr, idx) +
                    P::divi(half * (w0[iBY] + by_next) - half * (by_prev + w0[iBY]), dyr, idy);
         }
+        // fast path: divB c dt, with c the stage's weight on dt L(U); the
+        // centre cell of the two half-sums cancels: divB = (Bx_{i+1} -
+        // Bx_{i-1}) / (2 dx) + (By_{j+1} - By_{j-1}) / (2 dy)
+        real pw = real(0);
+        if constexpr (POWELL && !(P::exact || STAGE == 0)) pw = (wp1[iBX] - bxm) * pcx + (by_next - by_prev) * pcy;
         bxm = w0[iBX];
         real unew[NV];
         if constexpr (P::exact || STAGE == 0) {
@@ -363,7 +377,6 @@ __device__ __forceinline__ void stage_wave(const StageParams<real>& a, const int
             // into per-launch constants: U1 = U + dt L, U2 = 3/4 U0 + 1/4 U1
             // + (dt/4) L, U3 = 1/3 U0 + 2/3 U2 + (2 dt/3) L, L = -dFx/dx -
             // dFy/dy + S, each flux difference entering with one FMA.
-            const real pw = divb * cs;   // the Powell source enters as -divB (..) * c dt
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
                 const real fyl = from_prev(fy[v]);
@@ -387,10 +400,16 @@ __device__ __forceinline__ void stage_wave(const StageParams<real>& a, const int
         }
         if constexpr (STAGE >= 1) {
             // enforce_scalar_bounds (solver.py:199-204)
-            const real rho = unew[RHO] > real(FLOOR_RHO) ? unew[RHO] : real(FLOOR_RHO);
-            real c = P::div(unew[RHOC], rho);
-            c = c < real(0) ? real(0) : (c > real(1) ? real(1) : c);
-            unew[RHOC] = rho * c;
+            if constexpr (P::exact) {
+                const real rho = unew[RHO] > real(FLOOR_RHO) ? unew[RHO] : real(FLOOR_RHO);
+                real c = P::div(unew[RHOC], rho);
+                c = c < real(0) ? real(0) : (c > real(1) ? real(1) : c);
+                unew[RHOC] = rho * c;
+            } else {
+                const real rho = fmax(unew[RHO], real(FLOOR_RHO));
+                const real c = fmin(fmax(P::div(unew[RHOC], rho), real(0)), real(1));
+                unew[RHOC] = rho * c;
+            }
         }
         if constexpr (STAGE == 3) {
             if (ch_d > 0.0) unew[PSI] = unew[PSI] * real(decay_d);
