@@ -37,6 +37,14 @@ constexpr int STRIP = WAVE - 2 * HALO;  // output columns per wavefront
 constexpr int WPB = 4;                  // wavefronts per workgroup
 constexpr int NPART_DT = 4;             // {max cf, max(|vx|+|vy|), max|vx|, max|vy|}
 
+#ifdef AMHD_WAVE_TIMES
+// Diagnostic builds only (-DAMHD_WAVE_TIMES): start and end time of every
+// stage-kernel wavefront of the latest launch of each stage, in ticks of
+// the 100 MHz s_memrealtime clock, [stage][wave][start, end].
+constexpr int WT_WAVES = 8192;
+__device__ unsigned long long amhd_wave_times[4][WT_WAVES][2];
+#endif
+
 // Device-resident step control for the host-free time loop.
 struct StepCtl {
     double t, dt, ch, decay;
@@ -493,9 +501,19 @@ template <class real, class P, int STAGE, bool POWELL, bool GLM>
 __global__ __launch_bounds__(WAVE* WPB) AMHD_STAGE_OCC void stage_kernel(StageParams<real> a) {
     const int lane = threadIdx.x & (WAVE - 1);
     // wave-uniform quantities live in scalar registers
+#ifdef AMHD_WAVE_TIMES
+    const unsigned long long t_start = __builtin_amdgcn_s_memrealtime();
+#endif
     const int wave = __builtin_amdgcn_readfirstlane(xcd_logical_block() * WPB + (int)(threadIdx.x >> 6));
     if (wave >= a.n_strips * a.n_segs) return;
     stage_wave<real, P, STAGE, POWELL, GLM>(a, wave, lane);
+#ifdef AMHD_WAVE_TIMES
+    const unsigned long long t_end = __builtin_amdgcn_s_memrealtime();
+    if (lane == 0 && wave < WT_WAVES) {
+        amhd_wave_times[STAGE][wave][0] = t_start;
+        amhd_wave_times[STAGE][wave][1] = t_end;
+    }
+#endif
 }
 
 // ------------------------------------------------- CFL partials from state
