@@ -5,6 +5,7 @@
 // float with FastMath (contraction allowed).  amhd_api.hip only sees the
 // declarations below.
 #pragma once
+#include <algorithm>
 #include "amhd_kernels.hpp"
 
 namespace amhd {
@@ -92,17 +93,29 @@ hipError_t launch_batch_stage_impl(int stage, const BatchInst* insts, int n_inst
 
 template <class real, class P>
 int batch_waves_per_cu_impl() {
-    int nb = 0;
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_stage_kernel<real, P, 3>, WAVE * WPB, 0);
-    return e == hipSuccess && nb > 0 ? nb * WPB : WPB;
+    // resident wavefronts of the persistent launch: the fewest over its stages
+    auto blocks = [](auto kernel) {
+        int nb = 0;
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, WAVE * WPB, 0) == hipSuccess && nb > 0 ? nb : 1;
+    };
+    return std::min({blocks(batch_stage_kernel<real, P, 1>), blocks(batch_stage_kernel<real, P, 2>),
+                     blocks(batch_stage_kernel<real, P, 3>)}) * WPB;
 }
 
 template <class real, class P>
 int stage_waves_per_cu_impl(bool powell) {
-    int nb = 0;
-    hipError_t e = powell ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, stage_kernel<real, P, 2, true, true>, WAVE * WPB, 0)
-                          : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, stage_kernel<real, P, 2, false, true>, WAVE * WPB, 0);
-    return e == hipSuccess && nb > 0 ? nb * WPB : WPB;
+    // the three stages share one tiling: size it for the one with the fewest
+    // resident wavefronts (their register counts differ)
+    auto blocks = [](auto kernel) {
+        int nb = 0;
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, WAVE * WPB, 0) == hipSuccess && nb > 0 ? nb : 1;
+    };
+    const int nb = powell ? std::min({blocks(stage_kernel<real, P, 1, true, true>), blocks(stage_kernel<real, P, 2, true, true>),
+                                      blocks(stage_kernel<real, P, 3, true, true>)})
+                          : std::min({blocks(stage_kernel<real, P, 1, false, true>),
+                                      blocks(stage_kernel<real, P, 2, false, true>),
+                                      blocks(stage_kernel<real, P, 3, false, true>)});
+    return nb * WPB;
 }
 
 template <class real>
