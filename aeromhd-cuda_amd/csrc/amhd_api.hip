@@ -80,6 +80,7 @@ struct Ctx {
     amhd_params p{};
     int rows = 0, n_strips = 0, n_segs = 0;   // rows: requested rows per wave (0 = auto)
     int slots = 2048;                          // resident stage-kernel wavefronts on the device
+    int cus = 0;                               // compute units of the device (0 = unknown)
     long plane = 0;
     int pitch = 0;
     void* buf[3] = {nullptr, nullptr, nullptr};
@@ -126,6 +127,7 @@ StageParams<real> stage_args(const Ctx* c, int stage, const real* u0, const real
     a.part_bf = stage == 1 ? c->part_bf : nullptr;
     a.halo_lo = (const real*)c->halo[0];
     a.halo_hi = (const real*)c->halo[1];
+    a.first_blocks = c->cus;
     return a;
 }
 
@@ -291,6 +293,7 @@ int amhd_create(void** out, int nx, int ny, int dtype, const amhd_params* p) {
             const int wpc = dtype == AMHD_F64 ? stage_waves_per_cu_f64(p->powell != 0)
                                              : stage_waves_per_cu_f32(p->powell != 0);
             c->slots = cus * wpc;
+            c->cus = cus;
         }
     }
     set_tiling(c, 0);

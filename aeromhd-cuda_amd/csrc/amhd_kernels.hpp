@@ -75,6 +75,7 @@ template <class real> struct StageParams {
     // cells; null = the domain boundary (wrap / clamp, psi = 0).
     const real* halo_lo;
     const real* halo_hi;
+    int first_blocks;       // workgroups dispatched first onto each CU (the CU count); 0 = not known
 };
 
 // ---------------------------------------------------------- buffer access
@@ -185,7 +186,8 @@ __device__ __forceinline__ double bflux_term(const real (&u)[NV], double gamma, 
 // The work of one wavefront: strip `wave % n_strips`, segment
 // `wave / n_strips` of the grid described by `a`.  `wave` is wave-uniform.
 template <class real, class P, int STAGE, bool POWELL, bool GLM>
-__device__ __forceinline__ void stage_wave(const StageParams<real>& a, const int wave, const int lane) {
+__device__ __forceinline__ void stage_wave(const StageParams<real>& a, const int wave, const int lane,
+                                           const bool older = false) {
     const int strip = wave % a.n_strips;
     const int seg = wave / a.n_strips;
     const int nx = a.nx, ny = a.ny;
@@ -453,7 +455,11 @@ __device__ __forceinline__ void stage_wave(const StageParams<real>& a, const int
     // left: the one behind is preferred until it reaches the same mark, and
     // the wave that finishes first leads the other by at most about a row.
     constexpr bool PRIO = sizeof(real) == 4;
-    const int q1 = i0 + (i1 - i0) / 2, q2 = i1 - 2, q3 = i1 - 1;
+    // The wave dispatched first onto its CU (older: it wins ties) steps down
+    // one row earlier than the later one, which ends the two closer together
+    // (measured +1.6% against equal thresholds; two rows earlier was slower).
+    const int sh = older ? 1 : 0;
+    const int q1 = i0 + (i1 - i0) / 2 - sh, q2 = i1 - 2 - sh, q3 = i1 - 1 - sh;
     if constexpr (PRIO) __builtin_amdgcn_s_setprio(3);
     auto progress = [&](int done) __attribute__((always_inline)) {
         if constexpr (PRIO) {
@@ -506,7 +512,7 @@ __global__ __launch_bounds__(WAVE* WPB) AMHD_STAGE_OCC void stage_kernel(StagePa
 #endif
     const int wave = __builtin_amdgcn_readfirstlane(xcd_logical_block() * WPB + (int)(threadIdx.x >> 6));
     if (wave >= a.n_strips * a.n_segs) return;
-    stage_wave<real, P, STAGE, POWELL, GLM>(a, wave, lane);
+    stage_wave<real, P, STAGE, POWELL, GLM>(a, wave, lane, (int)blockIdx.x < a.first_blocks);
 #ifdef AMHD_WAVE_TIMES
     const unsigned long long t_end = __builtin_amdgcn_s_memrealtime();
     if (lane == 0 && wave < WT_WAVES) {
