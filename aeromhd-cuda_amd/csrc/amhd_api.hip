@@ -95,6 +95,7 @@ struct Ctx {
     hipStream_t stream = nullptr;
     hipGraphExec_t graph[3] = {nullptr, nullptr, nullptr};
     const void* halo[2] = {nullptr, nullptr};   // x-slab halo buffers (lo, hi), caller-owned
+    int* seg_i0 = nullptr;       // device [nx + 1]: first row of each segment, then nx
 };
 
 size_t esize(int dtype) { return dtype == AMHD_F32 ? sizeof(float) : sizeof(double); }
@@ -110,6 +111,51 @@ void set_tiling(Ctx* c, int rows) {
 }
 
 int n_waves(const Ctx* c) { return c->n_strips * c->n_segs; }
+
+// Segment boundaries of the tiling.  nx rows in n_segs segments are
+// nx % n_segs segments of one row more than the rest; the longer ones go to
+// the segments whose wavefronts belong to the workgroups dispatched first
+// onto each CU.  Those lead their CU's second workgroup (the SIMD issues
+// from the older wavefront on ties), so the longer segments there end with
+// the rest.  The mapping followed here is the stage kernel's: workgroup b
+// (dealt round-robin over the 8 XCDs, the first CU-count of them one per CU)
+// runs logical block xcd_logical_block(), whose WPB wavefronts w take
+// strip w % n_strips and segment w / n_strips.  Only the balance depends on
+// the dispatch order; any boundaries give the same result.
+std::vector<int> segment_rows(const Ctx* c) {
+    const int ns = c->n_strips, ng = c->n_segs, nx = c->nx;
+    const int base = nx / ng, n_long = nx - base * ng;
+    std::vector<int> first_waves(ng, 0);
+    const long waves = (long)ns * ng;
+    const long blocks = (waves + WPB - 1) / WPB;
+    const long nb8 = blocks & ~7L, per_xcd = nb8 / 8;
+    for (long b = 0; b < blocks; ++b) {
+        if (b >= c->cus) break;   // only the first-dispatched blocks count
+        const long lb = b < nb8 ? (b & 7) * per_xcd + (b >> 3) : b;
+        for (int k = 0; k < WPB; ++k) {
+            const long w = lb * WPB + k;
+            if (w < waves) first_waves[w / ns] += 1;
+        }
+    }
+    std::vector<int> order(ng);
+    for (int g = 0; g < ng; ++g) order[g] = g;
+    // most first-dispatched wavefronts first; ties spread over the segments
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+        if (first_waves[x] != first_waves[y]) return first_waves[x] > first_waves[y];
+        return (long)x * 7919 % ng < (long)y * 7919 % ng;
+    });
+    std::vector<int> len(ng, base);
+    for (int k = 0; k < n_long; ++k) len[order[k]] += 1;
+    std::vector<int> i0(ng + 1, 0);
+    for (int g = 0; g < ng; ++g) i0[g + 1] = i0[g] + len[g];
+    return i0;
+}
+
+hipError_t upload_segments(Ctx* c) {
+    const std::vector<int> i0 = segment_rows(c);
+    hipError_t e = hipMemcpyAsync(c->seg_i0, i0.data(), sizeof(int) * i0.size(), hipMemcpyHostToDevice, c->stream);
+    return e == hipSuccess ? hipStreamSynchronize(c->stream) : e;
+}
 
 template <class real>
 StageParams<real> stage_args(const Ctx* c, int stage, const real* u0, const real* us, real* uo) {
@@ -128,6 +174,7 @@ StageParams<real> stage_args(const Ctx* c, int stage, const real* u0, const real
     a.halo_lo = (const real*)c->halo[0];
     a.halo_hi = (const real*)c->halo[1];
     a.first_blocks = c->cus;
+    a.seg_i0 = c->seg_i0;
     return a;
 }
 
@@ -308,6 +355,8 @@ int amhd_create(void** out, int nx, int ny, int dtype, const amhd_params* p) {
     if (e == hipSuccess) e = hipMalloc(&c->part_bf, sizeof(double) * 2 * (size_t)c->n_strips);
     if (e == hipSuccess) e = hipMalloc(&c->gsum, sizeof(double) * 4 * GSUM_BLOCKS);
     if (e == hipSuccess) e = hipMalloc(&c->ctl, sizeof(StepCtl));
+    if (e == hipSuccess) e = hipMalloc(&c->seg_i0, sizeof(int) * (size_t)(nx + 1));
+    if (e == hipSuccess) e = upload_segments(c);
     if (e == hipSuccess) {
         StepCtl z{};
         e = hipMemcpyAsync(c->ctl, &z, sizeof z, hipMemcpyHostToDevice, c->stream);
@@ -334,6 +383,7 @@ int amhd_destroy(void* ctx) {
     if (c->part_bf) hipFree(c->part_bf);
     if (c->gsum) hipFree(c->gsum);
     if (c->ctl) hipFree(c->ctl);
+    if (c->seg_i0) hipFree(c->seg_i0);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
     return 0;
@@ -345,6 +395,7 @@ int amhd_set_rows(void* ctx, int rows) {
     Ctx* c = (Ctx*)ctx;
     HIP_TRY(hipStreamSynchronize(c->stream));
     set_tiling(c, rows);
+    HIP_TRY(upload_segments(c));
     drop_graphs(c);
     c->part_valid = false;   // per-wave partials follow the tiling
     return 0;

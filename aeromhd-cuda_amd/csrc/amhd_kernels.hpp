@@ -76,6 +76,7 @@ template <class real> struct StageParams {
     const real* halo_lo;
     const real* halo_hi;
     int first_blocks;       // workgroups dispatched first onto each CU (the CU count); 0 = not known
+    const int* seg_i0;      // [n_segs + 1] first row of each segment (null: seg*nx/n_segs)
 };
 
 // ---------------------------------------------------------- buffer access
@@ -191,8 +192,8 @@ __device__ __forceinline__ void stage_wave(const StageParams<real>& a, const int
     const int strip = wave % a.n_strips;
     const int seg = wave / a.n_strips;
     const int nx = a.nx, ny = a.ny;
-    const int i0 = (int)(((long)seg * nx) / a.n_segs);
-    const int i1 = (int)(((long)(seg + 1) * nx) / a.n_segs);
+    const int i0 = a.seg_i0 ? a.seg_i0[seg] : (int)(((long)seg * nx) / a.n_segs);
+    const int i1 = a.seg_i0 ? a.seg_i0[seg + 1] : (int)(((long)(seg + 1) * nx) / a.n_segs);
 
     double dt_d = a.dt, ch_d = a.ch, decay_d = a.decay;
     if (a.ctl) {
@@ -939,6 +940,8 @@ __global__ __launch_bounds__(WAVE* WPB) AMHD_STAGE_OCC void batch_stage_kernel(c
         a.part_bf = STAGE == 1 ? d.part_bf : nullptr;
         a.halo_lo = nullptr;
         a.halo_hi = nullptr;
+        a.first_blocks = 0;
+        a.seg_i0 = nullptr;
         const int wave = task - d.task_base;
         if (d.powell) stage_wave<real, P, STAGE, true, true>(a, wave, lane);
         else stage_wave<real, P, STAGE, false, true>(a, wave, lane);
