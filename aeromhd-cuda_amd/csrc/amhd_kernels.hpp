@@ -129,15 +129,15 @@ __device__ __forceinline__ void load_cell(const real* __restrict__ base, long pl
     for (int v = 0; v < NV; ++v) u[v] = base[v * plane];
 }
 
-// FP64 CFL contributions of one cell (solver.py:209-214), from conserved u:
+// CFL contributions of one cell (solver.py:209-214), from conserved u, in
+// the state's precision, as the reference computes them on its arrays:
 // m[0] collects cf^2 = gamma p/rho + B^2/rho (sqrt is monotone, so
 // sqrt(max cf^2) == max cf; the square root is taken once per step),
-// m[1] |vx|+|vy|, m[2] |vx|, m[3] |vy|, all in FP64 (an FP32 state is
-// widened first).  FP64 state: the reference's expression; FP32 state:
-// (gamma p + B^2) * (1/rho).
+// m[1] |vx|+|vy|, m[2] |vx|, m[3] |vy|.  FP64 state: the reference's
+// expression; FP32 state: (gamma p + B^2) * (1/rho) with one v_rcp_f32.
 template <class real>
-__device__ __forceinline__ void dt_terms(const real (&u)[NV], double gamma, double (&m)[NPART_DT]) {
-    double cf2, ax, ay;
+__device__ __forceinline__ void dt_terms(const real (&u)[NV], double gamma, real (&m)[NPART_DT]) {
+    real cf2, ax, ay;
     if constexpr (sizeof(real) == sizeof(double)) {
         // the reference's expressions (solver.py:208-214)
         double w[NV];
@@ -147,17 +147,16 @@ __device__ __forceinline__ void dt_terms(const real (&u)[NV], double gamma, doub
         ax = fabs(w[iVX]);
         ay = fabs(w[iVY]);
     } else {
-        // FP32 state widened to FP64; one IEEE division (1/rho), reused
-        const double rho = fmax(double(u[RHO]), FLOOR_RHO);
-        const double ri = 1.0 / rho;
-        const double vx = double(u[MX]) * ri, vy = double(u[MY]) * ri, vz = double(u[MZ]) * ri;
-        const double bx = double(u[BX]), by = double(u[BY]), bz = double(u[BZ]);
-        const double b2 = bx * bx + by * by + bz * bz;
-        const double p = fmax((gamma - 1.0) * (double(u[EN]) - 0.5 * rho * (vx * vx + vy * vy + vz * vz) - 0.5 * b2),
-                              FLOOR_PR);
-        cf2 = (gamma * p + b2) * ri;
-        ax = fabs(vx);
-        ay = fabs(vy);
+        const float g = float(gamma);
+        const float rho = fmaxf(u[RHO], float(FLOOR_RHO));
+        const float ri = __builtin_amdgcn_rcpf(rho);
+        const float vx = u[MX] * ri, vy = u[MY] * ri, vz = u[MZ] * ri;
+        const float b2 = u[BX] * u[BX] + u[BY] * u[BY] + u[BZ] * u[BZ];
+        const float rv2 = u[MX] * vx + u[MY] * vy + u[MZ] * vz;
+        const float p = fmaxf((g - 1.0f) * (u[EN] - 0.5f * (rv2 + b2)), float(FLOOR_PR));
+        cf2 = (g * p + b2) * ri;
+        ax = fabsf(vx);
+        ay = fabsf(vy);
     }
     m[0] = fmax(m[0], cf2);
     m[1] = fmax(m[1], ax + ay);
@@ -288,7 +287,7 @@ __device__ __forceinline__ void stage_wave(const StageParams<real>& a, const int
         bxm = wm1[iBX];
     }
 
-    double mx[NPART_DT] = {0.0, 0.0, 0.0, 0.0};
+    real mx[NPART_DT] = {0, 0, 0, 0};
     double bf_sum[2] = {0.0, 0.0};
     real upre[NV];
     load_in(i0 + 2, upre);
@@ -485,7 +484,7 @@ __device__ __forceinline__ void stage_wave(const StageParams<real>& a, const int
         if (a.part_dt) {
             double r[NPART_DT];
 #pragma unroll
-            for (int q = 0; q < NPART_DT; ++q) r[q] = wave_max(mx[q]);
+            for (int q = 0; q < NPART_DT; ++q) r[q] = wave_max(double(mx[q]));
             if (lane == 0) {
 #pragma unroll
                 for (int q = 0; q < NPART_DT; ++q) a.part_dt[(long)wave * NPART_DT + q] = r[q];
@@ -529,7 +528,7 @@ __global__ __launch_bounds__(WAVE* WPB) AMHD_STAGE_OCC void stage_kernel(StagePa
 template <class real>
 __global__ __launch_bounds__(256) void dt_partials_kernel(const real* __restrict__ u, long plane, int pitch,
                                                           int nx, int ny, double gamma, double* part) {
-    double m[NPART_DT] = {0.0, 0.0, 0.0, 0.0};
+    real m[NPART_DT] = {0, 0, 0, 0};
     const long n = (long)nx * ny;
     for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x) {
         const long i = k / ny, jj = k - i * ny;
@@ -541,7 +540,7 @@ __global__ __launch_bounds__(256) void dt_partials_kernel(const real* __restrict
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll
     for (int q = 0; q < NPART_DT; ++q) {
-        const double r = wave_max(m[q]);
+        const double r = wave_max(double(m[q]));
         if (lane == 0) sh[q][w] = r;
     }
     __syncthreads();
