@@ -393,45 +393,56 @@ __device__ __forceinline__ real hlld_face(const real (&wl)[NV], const real (&wr)
     // rho* >= floor_rho so the fast path multiplies by 1/sqrt(rho*)
     const real tk = P::divi(abx, fmax(sqk, real(1e-14)), rsqk);
     real sak;
-    if (useL) {
-        sak = sm - tk;
-        if (sak < sl) sak = sl;
-        if (sak > sm) sak = sm;
+    if constexpr (P::exact) {
+        if (useL) {
+            sak = sm - tk;
+            if (sak < sl) sak = sl;
+            if (sak > sm) sak = sm;
+        } else {
+            sak = sm + tk;
+            if (sak < sm) sak = sm;
+            if (sak > sr) sak = sr;
+        }
     } else {
-        sak = sm + tk;
-        if (sak < sm) sak = sm;
-        if (sak > sr) sak = sr;
+        // the same clamps as selects: min(max(SM - t, SL), SM) or min(max(SM + t, SM), SR)
+        sak = fmin(fmax(useL ? sm - tk : sm + tk, useL ? sl : sm), useL ? sm : sr);
     }
-    const real dds = fmax(sqLS + sqRS, real(1e-14));
-    const real idds = P::rcp(dds);
-    const real vyDS = P::divi(sqLS * vyLS + sqRS * vyRS + (byRS - byLS) * sbx, dds, idds);
-    const real vzDS = P::divi(sqLS * vzLS + sqRS * vzRS + (bzRS - bzLS) * sbx, dds, idds);
-    const real byDS = P::divi(sqLS * byRS + sqRS * byLS + sqLS * sqRS * (vyRS - vyLS) * sbx, dds, idds);
-    const real bzDS = P::divi(sqLS * bzRS + sqRS * bzLS + sqLS * sqRS * (vzRS - vzLS) * sbx, dds, idds);
-    const real vbDS = sm * bxf + vyDS * byDS + vzDS * bzDS;
-
-    const bool dbl = star && (useL ? (sak < real(0)) : (sak >= real(0)));
-    // E** (physics.py:171-172): L side subtracts, R side adds
-    const real eds = useL ? (es - sqk * (vbs - vbDS) * sbx) : (es + sqk * (vbs - vbDS) * sbx);
-
-    // physical flux of side K (physics.py:178-187)
-    real fk[NV];
-    fk[0] = rho * vx;
-    fk[1] = rho * vx * vx + pt - bx2;
-    fk[2] = rho * vy * vx - bxf * by;
-    fk[3] = rho * vz * vx - bxf * bz;
-    fk[4] = glm ? psif : real(0);
-    fk[5] = by * vx - bxf * vy;
-    fk[6] = bz * vx - bxf * vz;
-    fk[7] = (e + pt) * vx - bxf * vb;
-    fk[8] = glm ? bxf * k.ch * k.ch : real(0);
-    fk[9] = rho * c * vx;
-    // U_K, U*_K, U**_K (physics.py:213-249)
-    const real uk[NV] = {rho, rho * vx, rho * vy, rho * vz, bxf, by, bz, e, psi, rho * c};
-    const real us[NV] = {rhos, rhos * sm, rhos * vys, rhos * vzs, bxf, bys, bzs, es, psif, rhos * c};
-    const real uds[NV] = {rhos, rhos * sm, rhos * vyDS, rhos * vzDS, bxf, byDS, bzDS, eds, psif, rhos * c};
+    // physical flux of side K (physics.py:178-187) and U_K, U*_K
+    // (physics.py:213-249)
+    auto side_flux = [&](real (&fk)[NV], real (&uk)[NV], real (&us)[NV]) {
+        fk[0] = rho * vx;
+        fk[1] = rho * vx * vx + pt - bx2;
+        fk[2] = rho * vy * vx - bxf * by;
+        fk[3] = rho * vz * vx - bxf * bz;
+        fk[4] = glm ? psif : real(0);
+        fk[5] = by * vx - bxf * vy;
+        fk[6] = bz * vx - bxf * vz;
+        fk[7] = (e + pt) * vx - bxf * vb;
+        fk[8] = glm ? bxf * k.ch * k.ch : real(0);
+        fk[9] = rho * c * vx;
+        uk[0] = rho; uk[1] = rho * vx; uk[2] = rho * vy; uk[3] = rho * vz; uk[4] = bxf;
+        uk[5] = by; uk[6] = bz; uk[7] = e; uk[8] = psi; uk[9] = rho * c;
+        us[0] = rhos; us[1] = rhos * sm; us[2] = rhos * vys; us[3] = rhos * vzs; us[4] = bxf;
+        us[5] = bys; us[6] = bzs; us[7] = es; us[8] = psif; us[9] = rhos * c;
+    };
 
     if constexpr (P::exact) {
+        // double-star state (physics.py:156-172), formed for every face
+        const real dds = fmax(sqLS + sqRS, real(1e-14));
+        const real idds = P::rcp(dds);
+        const real vyDS = P::divi(sqLS * vyLS + sqRS * vyRS + (byRS - byLS) * sbx, dds, idds);
+        const real vzDS = P::divi(sqLS * vzLS + sqRS * vzRS + (bzRS - bzLS) * sbx, dds, idds);
+        const real byDS = P::divi(sqLS * byRS + sqRS * byLS + sqLS * sqRS * (vyRS - vyLS) * sbx, dds, idds);
+        const real bzDS = P::divi(sqLS * bzRS + sqRS * bzLS + sqLS * sqRS * (vzRS - vzLS) * sbx, dds, idds);
+        const real vbDS = sm * bxf + vyDS * byDS + vzDS * bzDS;
+
+        const bool dbl = star && (useL ? (sak < real(0)) : (sak >= real(0)));
+        // E** (physics.py:171-172): L side subtracts, R side adds
+        const real eds = useL ? (es - sqk * (vbs - vbDS) * sbx) : (es + sqk * (vbs - vbDS) * sbx);
+
+        real fk[NV], uk[NV], us[NV];
+        side_flux(fk, uk, us);
+        const real uds[NV] = {rhos, rhos * sm, rhos * vyDS, rhos * vzDS, bxf, byDS, bzDS, eds, psif, rhos * c};
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             if (GLM && (v == 4 || v == 8)) continue;   // set below
@@ -440,16 +451,35 @@ __device__ __forceinline__ real hlld_face(const real (&wl)[NV], const real (&wr)
             f[v] = star ? (dbl ? fds : fs) : fk[v];
         }
     } else {
-        // F* = F_K + S_K (U*_K - U_K); F** = F* + S*_K (U**_K - U*_K), the
-        // second term taken with S*_K = 0 outside the double-star region;
-        // components 0, 1, 4, 8, 9 of U** equal those of U*
+        // F* = F_K + S_K (U*_K - U_K) (S_K = 0 outside the star fan), then
+        // F** = F* + S*_K (U**_K - U*_K) for the faces in the double-star
+        // region only; components 0, 1, 4, 8, 9 of U** equal those of U*.
+        // A wavefront with no double-star face skips the second part.
+        real fk[NV], uk[NV], us[NV];
+        side_flux(fk, uk, us);
         const real sks = star ? sk : real(0);
-        const real sas = dbl ? sak : real(0);
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             if (GLM && (v == 4 || v == 8)) continue;   // set below
-            const real fs = fk[v] + sks * (us[v] - uk[v]);
-            f[v] = (v == 0 || v == 1 || v == 4 || v == 8 || v == 9) ? fs : fs + sas * (uds[v] - us[v]);
+            f[v] = fk[v] + sks * (us[v] - uk[v]);
+        }
+        const bool dbl = star && (useL ? (sak < real(0)) : (sak >= real(0)));
+        if (dbl) {
+            // double-star state (physics.py:156-172) and E** (physics.py:171-172)
+            const real dds = fmax(sqLS + sqRS, real(1e-14));
+            const real idds = P::rcp(dds);
+            const real vyDS = P::divi(sqLS * vyLS + sqRS * vyRS + (byRS - byLS) * sbx, dds, idds);
+            const real vzDS = P::divi(sqLS * vzLS + sqRS * vzRS + (bzRS - bzLS) * sbx, dds, idds);
+            const real byDS = P::divi(sqLS * byRS + sqRS * byLS + sqLS * sqRS * (vyRS - vyLS) * sbx, dds, idds);
+            const real bzDS = P::divi(sqLS * bzRS + sqRS * bzLS + sqLS * sqRS * (vzRS - vzLS) * sbx, dds, idds);
+            const real vbDS = sm * bxf + vyDS * byDS + vzDS * bzDS;
+            const real eds = useL ? (es - sqk * (vbs - vbDS) * sbx) : (es + sqk * (vbs - vbDS) * sbx);
+            const real uds[NV] = {rhos, rhos * sm, rhos * vyDS, rhos * vzDS, bxf, byDS, bzDS, eds, psif, rhos * c};
+#pragma unroll
+            for (int v = 2; v < 8; ++v) {
+                if (v == 4) continue;
+                f[v] = f[v] + sak * (uds[v] - us[v]);
+            }
         }
     }
     if (glm) {  // physics.py:265-268 (every glm face is in the star fan)
