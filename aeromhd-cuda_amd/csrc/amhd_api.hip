@@ -75,10 +75,22 @@ int get_scratch(Scratch** out) {
     return 0;
 }
 
+// Most segments of a narrow last strip per wavefront.  Each lane group of
+// a packed wavefront is on its own row, so its loads and stores touch a
+// cache line per group: measured on the 2048 x 1024 sieve (4 valid cells in
+// the last strip), packed wavefronts ran about 5% longer than the others
+// with 2 groups, 8% with 4 and 25% with 8 (stages 2 and 3, which also read
+// U0), which only 2 groups repay.
+#ifndef AMHD_PACK_MAX_GROUPS
+#define AMHD_PACK_MAX_GROUPS 2
+#endif
+
 struct Ctx {
     int nx = 0, ny = 0, dtype = AMHD_F64;
     amhd_params p{};
     int rows = 0, n_strips = 0, n_segs = 0;   // rows: requested rows per wave (0 = auto)
+    int pk_groups = 0, pk_shift = 0;           // packing of a narrow last strip (0 = none)
+    int pack_max = AMHD_PACK_MAX_GROUPS;       // most packed groups (amhd_set_packing; 0 = off)
     int slots = 2048;                          // resident stage-kernel wavefronts on the device
     int cus = 0;                               // compute units of the device (0 = unknown)
     long plane = 0;
@@ -98,19 +110,43 @@ struct Ctx {
     int* seg_i0 = nullptr;       // device [nx + 1]: first row of each segment, then nx
 };
 
+
 size_t esize(int dtype) { return dtype == AMHD_F32 ? sizeof(float) : sizeof(double); }
 
 // Strips of STRIP y cells; x rows split into n_segs balanced segments.
-// rows == 0: one segment per strip and resident wavefront slot, so that
-// every wave of a stage launch runs concurrently with equal work.
+// A last strip with few valid columns (r = ny - (n_strips - 1) STRIP <= 28)
+// is packed: groups of 64 / G lanes (r + 4 <= 64 / G; G = 2, 4 or 8, at
+// most the context's limit, by default 2) take G of its segments per wavefront, so it
+// costs n_segs / G wavefronts instead of n_segs.  rows == 0: as many segments as fill every resident
+// wavefront slot once, so that every wave of a stage launch runs
+// concurrently with equal work.
 void set_tiling(Ctx* c, int rows) {
     c->rows = rows;
     c->n_strips = (c->ny + STRIP - 1) / STRIP;
-    int segs = rows > 0 ? (c->nx + rows - 1) / rows : c->slots / c->n_strips;
-    c->n_segs = std::max(1, std::min(segs, c->nx));
+    const int r = c->ny - (c->n_strips - 1) * STRIP;
+    int g = 0;
+    if (c->pack_max) g = r + 2 * HALO <= 8 ? 8 : r + 2 * HALO <= 16 ? 4 : r + 2 * HALO <= 32 ? 2 : 0;
+    g = std::min(g, c->pack_max);
+    c->pk_groups = g;
+    c->pk_shift = g == 8 ? 3 : g == 4 ? 4 : g == 2 ? 5 : 0;
+    const long wide = g ? c->n_strips - 1 : c->n_strips;
+    long segs;
+    if (rows > 0) {
+        segs = (c->nx + rows - 1) / rows;
+    } else if (!g) {
+        segs = c->slots / c->n_strips;
+    } else {
+        // largest S with wide * S + ceil(S / g) <= slots
+        segs = (long)c->slots * g / (wide * g + 1);
+        while (segs > 1 && wide * segs + (segs + g - 1) / g > c->slots) --segs;
+    }
+    c->n_segs = (int)std::max(1L, std::min(segs, (long)c->nx));
 }
 
-int n_waves(const Ctx* c) { return c->n_strips * c->n_segs; }
+int n_waves(const Ctx* c) {
+    if (!c->pk_groups) return c->n_strips * c->n_segs;
+    return (c->n_strips - 1) * c->n_segs + (c->n_segs + c->pk_groups - 1) / c->pk_groups;
+}
 
 // Segment boundaries of the tiling.  nx rows in n_segs segments are
 // nx % n_segs segments of one row more than the rest; the longer ones go to
@@ -120,13 +156,15 @@ int n_waves(const Ctx* c) { return c->n_strips * c->n_segs; }
 // the rest.  The mapping followed here is the stage kernel's: workgroup b
 // (dealt round-robin over the 8 XCDs, the first CU-count of them one per CU)
 // runs logical block xcd_logical_block(), whose WPB wavefronts w take
-// strip w % n_strips and segment w / n_strips.  Only the balance depends on
-// the dispatch order; any boundaries give the same result.
+// strip w % ns and segment w / ns of the ns wide strips, and after those the
+// G segments of a packed wavefront.  Only the balance depends on the
+// dispatch order; any boundaries give the same result.
 std::vector<int> segment_rows(const Ctx* c) {
-    const int ns = c->n_strips, ng = c->n_segs, nx = c->nx;
+    const int ng = c->n_segs, nx = c->nx, g = c->pk_groups;
+    const int ns = g ? c->n_strips - 1 : c->n_strips;
     const int base = nx / ng, n_long = nx - base * ng;
     std::vector<int> first_waves(ng, 0);
-    const long waves = (long)ns * ng;
+    const long wide = (long)ns * ng, waves = n_waves(c);
     const long blocks = (waves + WPB - 1) / WPB;
     const long nb8 = blocks & ~7L, per_xcd = nb8 / 8;
     for (long b = 0; b < blocks; ++b) {
@@ -134,7 +172,9 @@ std::vector<int> segment_rows(const Ctx* c) {
         const long lb = b < nb8 ? (b & 7) * per_xcd + (b >> 3) : b;
         for (int k = 0; k < WPB; ++k) {
             const long w = lb * WPB + k;
-            if (w < waves) first_waves[w / ns] += 1;
+            if (w < wide) first_waves[w / ns] += 1;
+            else if (w < waves)
+                for (long s = (w - wide) * g; s < (w - wide + 1) * g && s < ng; ++s) first_waves[s] += 1;
         }
     }
     std::vector<int> order(ng);
@@ -175,6 +215,8 @@ StageParams<real> stage_args(const Ctx* c, int stage, const real* u0, const real
     a.halo_hi = (const real*)c->halo[1];
     a.first_blocks = c->cus;
     a.seg_i0 = c->seg_i0;
+    a.pk_groups = c->pk_groups;
+    a.pk_shift = c->pk_shift;
     return a;
 }
 
@@ -398,6 +440,20 @@ int amhd_set_rows(void* ctx, int rows) {
     HIP_TRY(upload_segments(c));
     drop_graphs(c);
     c->part_valid = false;   // per-wave partials follow the tiling
+    return 0;
+}
+
+int amhd_set_packing(void* ctx, int max_groups) {
+    if (!ctx) return fail(-1, "amhd_set_packing: null context");
+    if (max_groups != 0 && max_groups != 2 && max_groups != 4 && max_groups != 8)
+        return fail(-1, "amhd_set_packing: max_groups must be 0, 2, 4 or 8 (got %d)", max_groups);
+    Ctx* c = (Ctx*)ctx;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->pack_max = max_groups;
+    set_tiling(c, c->rows);
+    HIP_TRY(upload_segments(c));
+    drop_graphs(c);
+    c->part_valid = false;
     return 0;
 }
 

@@ -101,6 +101,14 @@ __device__ __forceinline__ double wave_max(double v) {
     }
     return v;
 }
+// Integer max over the 64 lanes, returned as a wave-uniform (scalar) value.
+__device__ __forceinline__ int wave_max_int(int v) {
+    for (int off = 32; off > 0; off >>= 1) {
+        const int o = __shfl_xor(v, off, 64);
+        v = o > v ? o : v;
+    }
+    return __builtin_amdgcn_readfirstlane(v);
+}
 __device__ __forceinline__ double wave_sum(double v) {
     // Fixed butterfly order: deterministic for a given lane assignment.
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);

@@ -64,7 +64,13 @@ template <class real> struct StageParams {
     int pitch, opitch;
     unsigned in_bytes, out_bytes;   // addressable bytes from u0/us and from uo
     int nx, ny, bcx_periodic, bcy_periodic;
-    int n_strips, n_segs;   // wave w: strip w % n_strips, rows [seg*nx/n_segs, (seg+1)*nx/n_segs)
+    int n_strips, n_segs;   // wave w: strip w % n_strips, segment w / n_strips (rows from seg_i0)
+    // Packing of the last strip when it is narrow (pk_groups = G in {2, 4, 8},
+    // 0 = none): the first (n_strips - 1) * n_segs wavefronts cover the other
+    // strips as above; each of the ceil(n_segs / G) after them covers G
+    // segments of the last strip, one per group of 64 / G lanes (2 halo lanes
+    // on each side), all marching in step.
+    int pk_groups, pk_shift;   // G, log2(64 / G)
     double gamma, dx, dy;
     double dt, ch, decay;   // used when ctl == nullptr
     const StepCtl* ctl;     // device-resident dt/ch/decay
@@ -79,10 +85,20 @@ template <class real> struct StageParams {
     const int* seg_i0;      // [n_segs + 1] first row of each segment (null: seg*nx/n_segs)
 };
 
+// Wavefronts of one stage launch: the wide strips' segments, then the
+// packed wavefronts of a narrow last strip.
+template <class real>
+__host__ __device__ inline int stage_wave_count(const StageParams<real>& a) {
+    if (!a.pk_groups) return a.n_strips * a.n_segs;
+    return (a.n_strips - 1) * a.n_segs + (a.n_segs + a.pk_groups - 1) / a.pk_groups;
+}
+
 // ---------------------------------------------------------- buffer access
 // Raw buffer loads/stores: one 32-bit per-lane offset (the column), the row
 // and variable plane go into the scalar offset, so a row of 10 variables
-// costs no vector address arithmetic.  Offsets past `bytes` read as 0.
+// costs no vector address arithmetic (a packed wavefront, whose lane groups
+// are on different rows, adds the row to the per-lane offset instead).
+// Offsets past `bytes` read as 0; stores there are dropped.
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* p, unsigned bytes) {
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
 }
@@ -185,14 +201,40 @@ __device__ __forceinline__ double bflux_term(const real (&u)[NV], double gamma, 
 // followed by the scalar bounds of solver.py:199-204 for stages 1-3.
 // The work of one wavefront: strip `wave % n_strips`, segment
 // `wave / n_strips` of the grid described by `a`.  `wave` is wave-uniform.
-template <class real, class P, int STAGE, bool POWELL, bool GLM>
+//
+// PK = false: strip `wave % n_wide`, segment `wave / n_wide`, with n_wide the
+// strips not packed (n_strips, or n_strips - 1); the row index is
+// wave-uniform.  PK = true: packed wavefront `wave` of the last
+// strip (StageParams::pk_groups); lane group g takes segment wave * G + g, so
+// the row index differs between groups and enters the buffer accesses
+// through the per-lane offset.  `part` is the wavefront's slot in the
+// per-wave partials.
+template <class real, class P, int STAGE, bool POWELL, bool GLM, bool PK = false>
 __device__ __forceinline__ void stage_wave(const StageParams<real>& a, const int wave, const int lane,
-                                           const bool older = false) {
-    const int strip = wave % a.n_strips;
-    const int seg = wave / a.n_strips;
+                                           const bool older = false, const int part = -1) {
     const int nx = a.nx, ny = a.ny;
+    int strip, seg, gl;
+    bool live = true;   // the lane's group has a segment (PK)
+    if constexpr (PK) {
+        strip = a.n_strips - 1;
+        gl = lane & ((1 << a.pk_shift) - 1);
+        seg = wave * a.pk_groups + (lane >> a.pk_shift);
+        live = seg < a.n_segs;
+        if (!live) seg = a.n_segs - 1;
+    } else {
+        // the wide strips: all of them, or all but a packed last one
+        const int n_wide = a.pk_groups ? a.n_strips - 1 : a.n_strips;
+        strip = wave % n_wide;
+        seg = wave / n_wide;
+        gl = lane;
+    }
+    const int gw = PK ? (1 << a.pk_shift) : WAVE;   // lanes per strip piece
     const int i0 = a.seg_i0 ? a.seg_i0[seg] : (int)(((long)seg * nx) / a.n_segs);
     const int i1 = a.seg_i0 ? a.seg_i0[seg + 1] : (int)(((long)(seg + 1) * nx) / a.n_segs);
+    // rows this wavefront steps through (wave-uniform); a PK lane whose
+    // segment is shorter idles through the last step
+    int n_rows = i1 - i0;
+    if constexpr (PK) n_rows = wave_max_int(live ? n_rows : 0);
 
     double dt_d = a.dt, ch_d = a.ch, decay_d = a.decay;
     if (a.ctl) {
@@ -211,10 +253,13 @@ __device__ __forceinline__ void stage_wave(const StageParams<real>& a, const int
     constexpr unsigned ES = sizeof(real);
     const unsigned pb = (unsigned)a.plane * ES;             // plane stride, bytes
     const unsigned rb = (unsigned)a.pitch * ES;             // row stride, bytes
-    const int j = strip * STRIP - HALO + lane;
+    const int j = strip * STRIP - HALO + gl;
     const int sj = remap(j, ny, a.bcy_periodic);
     const unsigned vo = (unsigned)sj * ES;
-    const bool out_lane = lane >= HALO && lane < WAVE - HALO && j < ny;
+    const bool out_lane = live && gl >= HALO && gl < gw - HALO && j < ny;
+    // buffer offsets of row si (row stride rs bytes): per-lane part, wave-uniform part
+    auto voff = [&](int si, unsigned rs) -> unsigned { return PK ? vo + (unsigned)si * rs : vo; };
+    auto soff = [&](int si, unsigned rs) -> unsigned { return PK ? 0u : (unsigned)si * rs; };
     const __amdgpu_buffer_rsrc_t r_us = make_rsrc(a.us, a.in_bytes);
     const __amdgpu_buffer_rsrc_t r_u0 = make_rsrc(a.u0, a.in_bytes);
     const __amdgpu_buffer_rsrc_t r_uo = make_rsrc(a.uo, a.out_bytes);
@@ -232,19 +277,25 @@ __device__ __forceinline__ void stage_wave(const StageParams<real>& a, const int
         return i;
     };
     auto load_row = [&](__amdgpu_buffer_rsrc_t r, int si, real (&u)[NV]) {
-        const unsigned so = (unsigned)si * rb;
+        const unsigned lo = voff(si, rb), so = soff(si, rb);
 #pragma unroll
-        for (int v = 0; v < NV; ++v) u[v] = bload(real(0), r, vo, so + (unsigned)v * pb);
+        for (int v = 0; v < NV; ++v) u[v] = bload(real(0), r, lo, so + (unsigned)v * pb);
     };
-    // stage-input row i for any i in [-2, nx + 2) (wave-uniform branch)
-    auto load_in = [&](int i, real (&u)[NV]) {
-        if ((i < 0 && hlo) || (i >= nx && hhi)) {
-            const __amdgpu_buffer_rsrc_t r = i < 0 ? r_hlo : r_hhi;
-            const unsigned so = (unsigned)(i < 0 ? i + 2 : i - nx) * rb;
+    auto load_halo = [&](__amdgpu_buffer_rsrc_t r, int hi, real (&u)[NV]) {
+        const unsigned lo = voff(hi, rb), so = soff(hi, rb);
 #pragma unroll
-            for (int v = 0; v < NV; ++v) u[v] = bload(real(0), r, vo, so + (unsigned)v * 2u * rb);
+        for (int v = 0; v < NV; ++v) u[v] = bload(real(0), r, lo, so + (unsigned)v * 2u * rb);
+    };
+    // stage-input row i for any i in [-2, nx + 2) (wave-uniform branch; per
+    // lane group with PK, each buffer resource staying wave-uniform)
+    auto load_in = [&](int i, real (&u)[NV]) {
+        if constexpr (PK) {
+            if (i < 0 && hlo) load_halo(r_hlo, i + 2, u);
+            else if (i >= nx && hhi) load_halo(r_hhi, i - nx, u);
+            else load_row(r_us, row_src(i), u);
         } else {
-            load_row(r_us, row_src(i), u);
+            if ((i < 0 && hlo) || (i >= nx && hhi)) load_halo(i < 0 ? r_hlo : r_hhi, i < 0 ? i + 2 : i - nx, u);
+            else load_row(r_us, row_src(i), u);
         }
     };
     auto psi_ghost = [&](int i) { return !xper && ((i < 0 && !hlo) || (i >= nx && !hhi)); };
@@ -425,16 +476,17 @@ __device__ __forceinline__ void stage_wave(const StageParams<real>& a, const int
             if (ch_d > 0.0) unew[PSI] = unew[PSI] * real(decay_d);
         }
 
-        if (out_lane) {
+        if (out_lane && (!PK || i < i1)) {
             if constexpr (STAGE == 0) {
-                const unsigned so = (unsigned)i * (unsigned)a.opitch * ES;
+                const unsigned ors = (unsigned)a.opitch * ES;
+                const unsigned lo = voff(i, ors), so = soff(i, ors);
                 const unsigned ob = (unsigned)a.oplane * ES;
 #pragma unroll
-                for (int v = 0; v < NV; ++v) bstore(unew[v], r_uo, vo, so + (unsigned)v * ob);
+                for (int v = 0; v < NV; ++v) bstore(unew[v], r_uo, lo, so + (unsigned)v * ob);
             } else {
-                const unsigned so = (unsigned)i * rb;
+                const unsigned lo = voff(i, rb), so = soff(i, rb);
 #pragma unroll
-                for (int v = 0; v < NV; ++v) bstore(unew[v], r_uo, vo, so + (unsigned)v * pb);
+                for (int v = 0; v < NV; ++v) bstore(unew[v], r_uo, lo, so + (unsigned)v * pb);
             }
             if constexpr (STAGE == 3) dt_terms(unew, a.gamma, mx);
             if constexpr (STAGE == 1) {
@@ -459,7 +511,7 @@ __device__ __forceinline__ void stage_wave(const StageParams<real>& a, const int
     // one row earlier than the later one, which ends the two closer together
     // (measured +1.6% against equal thresholds; two rows earlier was slower).
     const int sh = older ? 1 : 0;
-    const int q1 = i0 + (i1 - i0) / 2 - sh, q2 = i1 - 2 - sh, q3 = i1 - 1 - sh;
+    const int q1 = n_rows / 2 - sh, q2 = n_rows - 2 - sh, q3 = n_rows - 1 - sh;
     if constexpr (PRIO) __builtin_amdgcn_s_setprio(3);
     auto progress = [&](int done) __attribute__((always_inline)) {
         if constexpr (PRIO) {
@@ -468,16 +520,17 @@ __device__ __forceinline__ void stage_wave(const StageParams<real>& a, const int
             else if (done == q3) __builtin_amdgcn_s_setprio(0);
         }
     };
-    for (int i = i0;;) {
-        if (i >= i1) break;
-        row(i, A, B, C, QA, QC, F0, F1);
-        progress(++i);
-        if (i >= i1) break;
-        row(i, B, C, A, QB, QA, F1, F2);
-        progress(++i);
-        if (i >= i1) break;
-        row(i, C, A, B, QC, QB, F2, F0);
-        progress(++i);
+    // t: rows done (wave-uniform); the row itself is i0 + t
+    for (int t = 0;;) {
+        if (t >= n_rows) break;
+        row(i0 + t, A, B, C, QA, QC, F0, F1);
+        progress(++t);
+        if (t >= n_rows) break;
+        row(i0 + t, B, C, A, QB, QA, F1, F2);
+        progress(++t);
+        if (t >= n_rows) break;
+        row(i0 + t, C, A, B, QC, QB, F2, F0);
+        progress(++t);
     }
 
     if constexpr (STAGE == 3) {
@@ -485,9 +538,10 @@ __device__ __forceinline__ void stage_wave(const StageParams<real>& a, const int
             double r[NPART_DT];
 #pragma unroll
             for (int q = 0; q < NPART_DT; ++q) r[q] = wave_max(double(mx[q]));
+            const int slot = part >= 0 ? part : wave;
             if (lane == 0) {
 #pragma unroll
-                for (int q = 0; q < NPART_DT; ++q) a.part_dt[(long)wave * NPART_DT + q] = r[q];
+                for (int q = 0; q < NPART_DT; ++q) a.part_dt[(long)slot * NPART_DT + q] = r[q];
             }
         }
     }
@@ -495,9 +549,12 @@ __device__ __forceinline__ void stage_wave(const StageParams<real>& a, const int
         if (a.part_bf && !xper) {
             const double s0 = wave_sum(bf_sum[0]);
             const double s1 = wave_sum(bf_sum[1]);
+            // the wavefront holding the strip's first (last) segment
+            const bool first_seg = PK ? __builtin_amdgcn_ballot_w64(live && i0 == 0) != 0 : i0 == 0;
+            const bool last_seg = PK ? __builtin_amdgcn_ballot_w64(live && i1 == nx) != 0 : i1 == nx;
             if (lane == 0) {
-                if (i0 == 0) a.part_bf[strip * 2 + 0] = s0;
-                if (i1 == nx) a.part_bf[strip * 2 + 1] = s1;
+                if (first_seg) a.part_bf[strip * 2 + 0] = s0;
+                if (last_seg) a.part_bf[strip * 2 + 1] = s1;
             }
         }
     }
@@ -511,8 +568,14 @@ __global__ __launch_bounds__(WAVE* WPB) AMHD_STAGE_OCC void stage_kernel(StagePa
     const unsigned long long t_start = __builtin_amdgcn_s_memrealtime();
 #endif
     const int wave = __builtin_amdgcn_readfirstlane(xcd_logical_block() * WPB + (int)(threadIdx.x >> 6));
-    if (wave >= a.n_strips * a.n_segs) return;
-    stage_wave<real, P, STAGE, POWELL, GLM>(a, wave, lane, (int)blockIdx.x < a.first_blocks);
+    const bool older = (int)blockIdx.x < a.first_blocks;
+    const int n_wide = (a.pk_groups ? a.n_strips - 1 : a.n_strips) * a.n_segs;
+    if (wave < n_wide) {
+        stage_wave<real, P, STAGE, POWELL, GLM>(a, wave, lane, older);
+    } else if constexpr (STAGE >= 1) {
+        if (a.pk_groups && wave < stage_wave_count(a))
+            stage_wave<real, P, STAGE, POWELL, GLM, true>(a, wave - n_wide, lane, older, wave);
+    }
 #ifdef AMHD_WAVE_TIMES
     const unsigned long long t_end = __builtin_amdgcn_s_memrealtime();
     if (lane == 0 && wave < WT_WAVES) {
@@ -941,6 +1004,8 @@ __global__ __launch_bounds__(WAVE* WPB) AMHD_STAGE_OCC void batch_stage_kernel(c
         a.halo_hi = nullptr;
         a.first_blocks = 0;
         a.seg_i0 = nullptr;
+        a.pk_groups = 0;
+        a.pk_shift = 0;
         const int wave = task - d.task_base;
         if (d.powell) stage_wave<real, P, STAGE, true, true>(a, wave, lane);
         else stage_wave<real, P, STAGE, false, true>(a, wave, lane);

@@ -66,7 +66,7 @@ void launch_stage_t(bool powell, bool glm, const StageParams<real>& a, dim3 grid
 // has ch >= 1.5; host-driven steps pass their ch).
 template <class real, class P>
 hipError_t launch_stage_impl(int stage, bool powell, bool glm, const StageParams<real>& a, hipStream_t s) {
-    const int n_waves = a.n_strips * a.n_segs;
+    const int n_waves = stage_wave_count(a);
     const dim3 grid((n_waves + WPB - 1) / WPB), block(WAVE * WPB);
     switch (stage) {
         case 0: launch_stage_t<real, P, 0>(powell, glm, a, grid, block, s); break;

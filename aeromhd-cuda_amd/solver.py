@@ -87,6 +87,12 @@ class Context:
     def rows(self) -> int:
         return int(self.lib.amhd_get_rows(self.h))
 
+    def set_packing(self, max_groups: int):
+        """Let up to max_groups (2, 4 or 8; 0 = off; default 2) segments of a
+        narrow last y strip share one wavefront; the results do not depend
+        on it."""
+        _L.check(self.lib.amhd_set_packing(self.h, int(max_groups)), "amhd_set_packing")
+
     def upload(self, U) -> None:
         torch = _L.require_gpu()
         if isinstance(U, torch.Tensor):

@@ -60,6 +60,8 @@ def summarize(t: np.ndarray, n_waves: int, cus_per_xcd: int) -> dict:
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--detail", type=int, default=0,
+                    help="also list the N slowest wavefronts and the last N (packed ones come last)")
     args = ap.parse_args()
     import amhd_lib as L
     torch = L.require_gpu()
@@ -76,6 +78,14 @@ def main():
     for stage in (1, 2, 3):
         n = int((t[stage, :, 1] > 0).sum())
         print(f"stage {stage}: {n} wavefronts", summarize(t[stage], n, cus_per_xcd), flush=True)
+        if args.detail:
+            tt = t[stage, :n]
+            dur = (tt[:, 1] - tt[:, 0]) / 100.0
+            t0 = tt[:, 0].min()
+            slow = np.argsort(dur)[::-1][:args.detail]
+            fmt = lambda ids: [(int(w), round(float(dur[w]), 1), round(float(tt[w, 0] - t0) / 100.0, 1)) for w in ids]
+            print("  slowest (wave, us, start us):", fmt(slow), flush=True)
+            print("  last (wave, us, start us):", fmt(range(max(0, n - args.detail), n)), flush=True)
 
 
 if __name__ == "__main__":

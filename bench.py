@@ -56,7 +56,7 @@ def make_workload(name: str, seed: int):
     return cfg, W, desc
 
 
-def gpu_measure(name: str, steps: int, warmup: int, seed: int, rows: int | None):
+def gpu_measure(name: str, steps: int, warmup: int, seed: int, rows: int | None, packing: int | None = None):
     import torch
     from physics import prim_to_cons
     from solver import MHDSolver
@@ -65,6 +65,8 @@ def gpu_measure(name: str, steps: int, warmup: int, seed: int, rows: int | None)
     s = MHDSolver(cfg)
     if rows:
         s.ctx.set_rows(rows)
+    if packing is not None:
+        s.ctx.set_packing(packing)
     s.U = prim_to_cons(W, cfg.gamma)
     s.advance(warmup)
     # timed region 1 (value): the product loop -- device dt, hipGraph replay
@@ -291,6 +293,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="sieve2048")
     ap.add_argument("--rows", type=int, default=0)
+    ap.add_argument("--packing", type=int, default=None,
+                    help="most segments of a narrow last y strip per wavefront: 0 (off), 2, 4 or 8")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-steps", type=int, default=16)
     ap.add_argument("--cpu-steps-1core", type=int, default=2)
@@ -310,7 +314,7 @@ def main():
     if args.copy_calib:
         copy_calibration()
     log(f"[bench] workload {args.workload}: warmup {args.warmup}, steps {args.steps}")
-    m = gpu_measure(args.workload, args.steps, args.warmup, args.seed, args.rows or None)
+    m = gpu_measure(args.workload, args.steps, args.warmup, args.seed, args.rows or None, args.packing)
     if not m["finite"]:
         print(f"bench.py: the {args.workload} state is not finite after the timed steps", file=sys.stderr)
         sys.exit(3)

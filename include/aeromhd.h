@@ -60,6 +60,13 @@ int amhd_destroy(void* ctx);
 int amhd_set_rows(void* ctx, int rows);
 int amhd_get_rows(void* ctx);
 
+/* Packing of a narrow last y strip: when the last strip of 60 columns
+ * holds at most 28 cells, up to max_groups of its x segments (2, 4 or 8;
+ * default 2; 0 = no packing) share one wavefront.  A tuning switch like
+ * amhd_set_rows (no counterpart in the reference); results do not depend
+ * on it. */
+int amhd_set_packing(void* ctx, int max_groups);
+
 /* MHDSolver.U assignment / read (solver.py:83, 139).  U: ghosted float64
  * (10, nx+4, ny+4), host or device memory.  On read, ghost cells are filled
  * as apply_bc (solver.py:158-197) leaves them. */

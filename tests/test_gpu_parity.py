@@ -183,16 +183,56 @@ def test_single_step_bitwise_with_tilings(torch_gpu, orc):
     U = prim_to_cons_np(smooth_state(97, 131, seed=9, amp=0.3), cfg.gamma)
     dt, ch = orc.compute_dt(orc.cons_to_prim(U, cfg.gamma), cfg)
     Uo, bfo = orc.step(U, cfg, dt, ch)
-    for rows in (1, 3, 16, 97):
-        say("single step rows", rows)
+    # ny = 131: the last strip holds 11 cells, packed 2 or 4 segments to a wavefront
+    for rows, pack in ((1, 2), (3, 4), (16, 0), (97, 4)):
+        say("single step rows", rows, "packing", pack)
         s = MHDSolver(cfg)
         s.ctx.set_rows(rows)
+        s.ctx.set_packing(pack)
         s.U = U
         s.dt, cfg.glm_ch = dt, ch
         s._cumulative_boundary_energy = 0.0
         s.step_ssprk3()
         np.testing.assert_array_equal(s.U.cpu().numpy(), Uo)
         assert s._cumulative_boundary_energy == pytest.approx(bfo * dt, rel=1e-12, abs=1e-300)
+
+
+@pytest.mark.parametrize("precision,nx,ny,groups", [("fp32", 512, 1024, 8), ("fp32", 512, 1024, 2),
+                                                    ("fp32", 300, 88, 8), ("fp64", 200, 68, 4),
+                                                    ("fp64", 64, 4, 8), ("fp64", 64, 4, 2)])
+def test_packed_last_strip_is_bitwise_neutral(torch_gpu, precision, nx, ny, groups):
+    """A last y strip with 4, 28, 8 or 4 valid cells packed up to `groups`
+    segments to a wavefront (8, 4 or 2 lane groups as the strip allows; ny =
+    4: every wavefront is packed).  15 device-loop steps of the magnetic
+    sieve (extrapolated x, so the boundary energy flux is summed too) give
+    the same state, time and boundary energy bit for bit as with packing
+    off."""
+    from amhd_ics import cons_from_prim_host, magnetic_sieve
+    from solver import MHDSolver
+
+    cfg, W = magnetic_sieve(nx=nx, ny=ny, seed=3, precision=precision, max_steps=10 ** 6)
+    U0 = cons_from_prim_host(W, cfg.gamma)
+    out = []
+    for pack in (groups, 0):
+        say(f"packing {pack}: {precision} {nx}x{ny}")
+        s = MHDSolver(cfg)
+        s.ctx.set_packing(pack)
+        s.U = U0
+        assert s.advance(15) == 15
+        out.append((s.ctx.download_numpy(), s.t, s._cumulative_boundary_energy))
+    assert np.isfinite(out[0][0]).all()
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    assert out[0][1] == out[1][1] and out[0][2] == out[1][2]
+
+
+def test_packing_limit_is_checked(torch_gpu):
+    from config import Config
+    from solver import MHDSolver
+
+    s = MHDSolver(Config(nx=16, ny=64))
+    for bad in (1, 3, 16, -2):
+        with pytest.raises(Exception, match="max_groups"):
+            s.ctx.set_packing(bad)
 
 
 def test_single_step_bitwise_y_outflow(torch_gpu, orc):
