@@ -75,12 +75,13 @@ int get_scratch(Scratch** out) {
     return 0;
 }
 
-// Most segments of a narrow last strip per wavefront.  Each lane group of
-// a packed wavefront is on its own row, so its loads and stores touch a
-// cache line per group: measured on the 2048 x 1024 sieve (4 valid cells in
-// the last strip), packed wavefronts ran about 5% longer than the others
-// with 2 groups, 8% with 4 and 25% with 8 (stages 2 and 3, which also read
-// U0), which only 2 groups repay.
+// Most segments of a narrow last strip per wavefront (the default where
+// two or more stage wavefronts share a SIMD).  Each lane group of a packed
+// wavefront is on its own row, so its loads and stores touch a cache line
+// per group: measured on the 2048 x 1024 FP32 sieve (4 valid cells in the
+// last strip), packed wavefronts ran about 5% longer than the others with 2
+// groups, 8% with 4 and 25% with 8 (stages 2 and 3, which also read U0),
+// which only 2 groups repay.
 #ifndef AMHD_PACK_MAX_GROUPS
 #define AMHD_PACK_MAX_GROUPS 2
 #endif
@@ -116,9 +117,10 @@ size_t esize(int dtype) { return dtype == AMHD_F32 ? sizeof(float) : sizeof(doub
 // Strips of STRIP y cells; x rows split into n_segs balanced segments.
 // A last strip with few valid columns (r = ny - (n_strips - 1) STRIP <= 28)
 // is packed: groups of 64 / G lanes (r + 4 <= 64 / G; G = 2, 4 or 8, at
-// most the context's limit, by default 2) take G of its segments per wavefront, so it
-// costs n_segs / G wavefronts instead of n_segs.  rows == 0: as many segments as fill every resident
-// wavefront slot once, so that every wave of a stage launch runs
+// most the context's limit: 2 by default with two wavefronts per SIMD, else
+// 0) take G of its segments per wavefront, so it costs n_segs / G
+// wavefronts instead of n_segs.  rows == 0: as many segments as fill every
+// resident wavefront slot once, so that every wave of a stage launch runs
 // concurrently with equal work.
 void set_tiling(Ctx* c, int rows) {
     c->rows = rows;
@@ -383,6 +385,10 @@ int amhd_create(void** out, int nx, int ny, int dtype, const amhd_params* p) {
                                              : stage_waves_per_cu_f32(p->powell != 0);
             c->slots = cus * wpc;
             c->cus = cus;
+            // packing pays only where a second wavefront on the SIMD hides
+            // the packed wavefront's longer memory instructions (measured on
+            // the 2048 x 1024 sieve: FP32, 2 per SIMD, +1%; FP64, 1 per SIMD, -10%)
+            if (wpc < 8) c->pack_max = 0;
         }
     }
     set_tiling(c, 0);

@@ -88,9 +88,9 @@ class Context:
         return int(self.lib.amhd_get_rows(self.h))
 
     def set_packing(self, max_groups: int):
-        """Let up to max_groups (2, 4 or 8; 0 = off; default 2) segments of a
-        narrow last y strip share one wavefront; the results do not depend
-        on it."""
+        """Let up to max_groups (2, 4 or 8; 0 = off) segments of a narrow
+        last y strip share one wavefront (default: 2 for FP32 contexts, 0 for
+        FP64); the results do not depend on it."""
         _L.check(self.lib.amhd_set_packing(self.h, int(max_groups)), "amhd_set_packing")
 
     def upload(self, U) -> None:
