@@ -172,16 +172,23 @@ __device__ __forceinline__ real vl_slope(real wm, real w0, real wp) {
 // (physics.py:329-336) from the cell value.  The fast path forms
 // ab / (dm + dp) directly, which equals 0.5 * (2 ab / (..)) exactly (scaling
 // by 2 and by 1/2 is exact), and drops the 1e-30 guard: where ab > 0, dm
-// and dp are nonzero with one sign, so dm + dp != 0; where ab <= 0 the
-// quotient (possibly inf or NaN) is not selected.
+// and dp are nonzero with one sign, so dm + dp != 0.  The select ab > 0 ?
+// .. : 0 becomes max(ab, 0) * (1 / (dm + dp)) with the legacy multiply,
+// for which 0 * x = 0 for every x (inf and NaN included): a flat stencil
+// (dm = dp = 0, reciprocal inf) or an extremum gives 0, and ab > 0 gives
+// the ordinary IEEE product.
+// (v_mul_legacy_f32 through its LLVM intrinsic, so that the compiler
+// schedules it and inserts the wait states after a v_rcp_f32 it consumes)
+extern "C" __device__ float amhd_fmul_legacy(float, float) __asm("llvm.amdgcn.fmul.legacy");
+__device__ __forceinline__ float mul_legacy(float a, float b) { return amhd_fmul_legacy(a, b); }
+
 template <class real, class P>
 __device__ __forceinline__ real vl_half_d(real dm, real dp) {
     if constexpr (P::exact) {
         return real(0.5) * vl_slope_d<real, P>(dm, dp);
     } else {
         const real ab = dm * dp;
-        const real h = ab * P::rcp(dm + dp);
-        return ab > real(0) ? h : real(0);
+        return mul_legacy(fmaxf(ab, 0.0f), P::rcp(dm + dp));
     }
 }
 
@@ -307,8 +314,14 @@ __device__ __forceinline__ real hlld_face(const real (&wl)[NV], const real (&wr)
     real sm = P::div(dsR * rhoR * vxR - dsL * rhoL * vxL - ptR + ptL, den);
     {
         const real e = P::eps_sm(sl, sr);
-        if (sm < sl + e) sm = sl + e;
-        if (sm > sr - e) sm = sr - e;
+        if constexpr (P::exact) {
+            if (sm < sl + e) sm = sl + e;
+            if (sm > sr - e) sm = sr - e;
+        } else {
+            // one median: SL + e <= SR - e, as e = max(1e-14, 2^-20 (SR - SL)) and
+            // SR - SL >= cf_L + cf_R > 0
+            sm = __builtin_amdgcn_fmed3f(sm, sl + e, sr - e);
+        }
     }
     const real pts = ptL + rhoL * dsL * (sm - vxL);
 
@@ -333,8 +346,14 @@ __device__ __forceinline__ real hlld_face(const real (&wl)[NV], const real (&wr)
     const real eps_rel = P::eps_rel_scale() * fmax(fabs(pts), real(1));
     real dnL = rhoL * dsL * dslsm - bx2;
     real dnR = rhoR * dsR * dsrsm - bx2;
-    if (fabs(dnL) < eps_rel) dnL = (dnL >= real(0)) ? eps_rel : -eps_rel;
-    if (fabs(dnR) < eps_rel) dnR = (dnR >= real(0)) ? eps_rel : -eps_rel;
+    if constexpr (P::exact) {
+        if (fabs(dnL) < eps_rel) dnL = (dnL >= real(0)) ? eps_rel : -eps_rel;
+        if (fabs(dnR) < eps_rel) dnR = (dnR >= real(0)) ? eps_rel : -eps_rel;
+    } else {
+        // the same guard as max and a sign copy (it differs only for D = -0)
+        dnL = copysignf(fmaxf(fabsf(dnL), eps_rel), dnL);
+        dnR = copysignf(fmaxf(fabsf(dnR), eps_rel), dnR);
+    }
     real idnL, idnR;
     P::rcp2(dnL, dnR, idnL, idnR);
 
@@ -412,8 +431,10 @@ __device__ __forceinline__ real hlld_face(const real (&wl)[NV], const real (&wr)
             if (sak > sr) sak = sr;
         }
     } else {
-        // the same clamps as selects: min(max(SM - t, SL), SM) or min(max(SM + t, SM), SR)
-        sak = fmin(fmax(useL ? sm - tk : sm + tk, useL ? sl : sm), useL ? sm : sr);
+        // the same clamps as one median: SL <= SM - t <= SM <= SR for K = L and
+        // SL <= SM <= SM + t for K = R (t >= 0), so min(max(SM - t, SL), SM) and
+        // min(max(SM + t, SM), SR) are both med3(SM -/+ t, SL, SR)
+        sak = __builtin_amdgcn_fmed3f(sm + (useL ? -tk : tk), sl, sr);
     }
     // physical flux of side K (physics.py:178-187) and U_K, U*_K
     // (physics.py:213-249)
@@ -471,7 +492,11 @@ __device__ __forceinline__ real hlld_face(const real (&wl)[NV], const real (&wr)
             if (GLM && (v == 4 || v == 8)) continue;   // set below
             f[v] = fk[v] + sks * (us[v] - uk[v]);
         }
-        const bool dbl = star && (useL ? (sak < real(0)) : (sak >= real(0)));
+        // S*_K < 0 (K = L) or >= 0 (K = R).  At S*_K = 0 the correction
+        // below vanishes, so both sides may leave it out there: the test is
+        // (K = L ? -S*_K : S*_K) > 0, and a face with Bx = 0 and SM = 0 (a
+        // uniform state at rest) skips the branch
+        const bool dbl = star && (useL ? -sak : sak) > real(0);
         if (dbl) {
             // double-star state (physics.py:156-172) and E** (physics.py:171-172)
             const real dds = fmax(sqLS + sqRS, real(1e-14));
