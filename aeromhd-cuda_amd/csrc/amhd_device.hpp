@@ -459,28 +459,37 @@ __device__ __forceinline__ real hlld_face(const real (&wl)[NV], const real (&wr)
     };
 
     if constexpr (P::exact) {
-        // double-star state (physics.py:156-172), formed for every face
-        const real dds = fmax(sqLS + sqRS, real(1e-14));
-        const real idds = P::rcp(dds);
-        const real vyDS = P::divi(sqLS * vyLS + sqRS * vyRS + (byRS - byLS) * sbx, dds, idds);
-        const real vzDS = P::divi(sqLS * vzLS + sqRS * vzRS + (bzRS - bzLS) * sbx, dds, idds);
-        const real byDS = P::divi(sqLS * byRS + sqRS * byLS + sqLS * sqRS * (vyRS - vyLS) * sbx, dds, idds);
-        const real bzDS = P::divi(sqLS * bzRS + sqRS * bzLS + sqLS * sqRS * (vzRS - vzLS) * sbx, dds, idds);
-        const real vbDS = sm * bxf + vyDS * byDS + vzDS * bzDS;
-
+        // F = F_K outside the star fan, F*_K = F_K + S_K (U*_K - U_K) in it,
+        // and F**_K = F*_K + S*_K (U**_K - U*_K) in the double-star region
+        // (physics.py:178-275).  U** is formed only where a face is in that
+        // region, with the same operations as the reference on the same
+        // values, so every face's flux is the same bit for bit.
         const bool dbl = star && (useL ? (sak < real(0)) : (sak >= real(0)));
-        // E** (physics.py:171-172): L side subtracts, R side adds
-        const real eds = useL ? (es - sqk * (vbs - vbDS) * sbx) : (es + sqk * (vbs - vbDS) * sbx);
-
         real fk[NV], uk[NV], us[NV];
         side_flux(fk, uk, us);
-        const real uds[NV] = {rhos, rhos * sm, rhos * vyDS, rhos * vzDS, bxf, byDS, bzDS, eds, psif, rhos * c};
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             if (GLM && (v == 4 || v == 8)) continue;   // set below
             const real fs = fk[v] + sk * (us[v] - uk[v]);
-            const real fds = fs + sak * (uds[v] - us[v]);
-            f[v] = star ? (dbl ? fds : fs) : fk[v];
+            f[v] = star ? fs : fk[v];
+        }
+        if (dbl) {
+            // double-star state (physics.py:156-172)
+            const real dds = fmax(sqLS + sqRS, real(1e-14));
+            const real idds = P::rcp(dds);
+            const real vyDS = P::divi(sqLS * vyLS + sqRS * vyRS + (byRS - byLS) * sbx, dds, idds);
+            const real vzDS = P::divi(sqLS * vzLS + sqRS * vzRS + (bzRS - bzLS) * sbx, dds, idds);
+            const real byDS = P::divi(sqLS * byRS + sqRS * byLS + sqLS * sqRS * (vyRS - vyLS) * sbx, dds, idds);
+            const real bzDS = P::divi(sqLS * bzRS + sqRS * bzLS + sqLS * sqRS * (vzRS - vzLS) * sbx, dds, idds);
+            const real vbDS = sm * bxf + vyDS * byDS + vzDS * bzDS;
+            // E** (physics.py:171-172): L side subtracts, R side adds
+            const real eds = useL ? (es - sqk * (vbs - vbDS) * sbx) : (es + sqk * (vbs - vbDS) * sbx);
+            const real uds[NV] = {rhos, rhos * sm, rhos * vyDS, rhos * vzDS, bxf, byDS, bzDS, eds, psif, rhos * c};
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                if (GLM && (v == 4 || v == 8)) continue;   // set below
+                f[v] = f[v] + sak * (uds[v] - us[v]);   // dbl implies star: f[v] is F*_K
+            }
         }
     } else {
         // F* = F_K + S_K (U*_K - U_K) (S_K = 0 outside the star fan), then
