@@ -975,11 +975,29 @@ struct Batch {
     hipGraphExec_t graph = nullptr;
 };
 
+// Tasks of the batch with r rows per segment.
+long batch_tasks(const Batch* b, int r) {
+    long t = 0;
+    for (int k = 0; k < b->n; ++k)
+        t += (long)((b->ny[k] + STRIP - 1) / STRIP) * std::max(1, std::min((b->nx[k] + r - 1) / r, b->nx[k]));
+    return t;
+}
+
 int batch_layout(Batch* b, int rows) {
     b->rows = rows;
-    long total = 0;
-    for (int k = 0; k < b->n; ++k) total += (long)b->nx[k] * ((b->ny[k] + STRIP - 1) / STRIP);
-    int r = rows > 0 ? rows : (int)std::max<long>(1, (total + b->resident - 1) / b->resident);
+    int r = rows;
+    if (r <= 0) {
+        // The resident wavefronts take the tasks round by round, so a step
+        // lasts about ceil(tasks / resident) tasks of r rows plus the
+        // segment prologue (about 0.6 of a row: two extra primitive rows
+        // and one extra x face).  Take the r in [4, 64] that minimises it.
+        double best = 1e300;
+        for (int c = 4; c <= 64; ++c) {
+            const long rounds = (batch_tasks(b, c) + b->resident - 1) / b->resident;
+            const double cost = (double)rounds * (c + 0.6);
+            if (cost < best) { best = cost; r = c; }
+        }
+    }
     int base = 0;
     for (int k = 0; k < b->n; ++k) {
         BatchInst& d = b->inst[k];

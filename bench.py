@@ -119,7 +119,7 @@ def batch_measure(n: int, steps: int, warmup: int, seed: int, seq_steps: int = 2
         cases.append((cfg, cons_from_prim_host(W, cfg.gamma)))
     del insts
     b = BatchSolver(cases)
-    rows = amhd_tune.tune_batch(b, candidates=(0, 4, 8, 16), steps=3, use_cache=False)
+    rows = amhd_tune.tune_batch(b, candidates=(0, 8, 12, 16, 20, 26, 32), steps=3, use_cache=False)
     b.run(warmup)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
