@@ -307,7 +307,7 @@ def main():
     ap.add_argument("--no-closure", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               "(the single-instance solver does not shard across GPUs).", file=sys.stderr)
         sys.exit(2)
 
