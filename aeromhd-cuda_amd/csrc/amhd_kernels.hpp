@@ -657,14 +657,76 @@ __global__ __launch_bounds__(256) void dt_partials_w_kernel(const double* __rest
 // determine dt bit for bit).  Also closes a step: boundary-flux sum and its
 // accumulation (solver.py:316-317), t and step advance (solver.py:481-482).
 // One workgroup of 256 threads.  CTL_DEVICE_LOOP reads cfl from ctl.
+// The step-control update of step_ctl_block on a register copy of StepCtl.
+__device__ __forceinline__ void step_ctl_update(StepCtl& h, const double (&r)[NPART_DT], double bflux, double dx,
+                                                double dy, double host_dt, double cfl, int mode) {
+    if (mode == CTL_HOST_STEP_END) {
+        h.bflux = bflux;
+        h.bflux_cum += bflux * host_dt;
+        return;
+    }
+    if (mode == CTL_DEVICE_LOOP && h.pending) {
+        h.bflux = bflux;
+        h.bflux_cum += bflux * h.dt;
+        h.t += h.dt;
+        h.steps += 1;
+        h.cur = (h.cur + 1) % 3;
+        h.pending = 0;
+    }
+    if (mode == CTL_DEVICE_LOOP) {
+        if (h.done) return;
+        cfl = h.cfl;
+    }
+
+    const double cfmax = sqrt(r[0]);   // partials hold max cf^2
+    const double ch = fmax(cfmax + r[1], 1.0) * 1.5;
+    const double sx = r[2] + ch, sy = r[3] + ch;
+    const double sm = fmax(sx / dx, sy / dy);
+    double dt = (sm < 1e-14) ? cfl * fmin(dx, dy) : cfl / sm;
+    h.cfmax = cfmax;
+    h.vmax = r[1];
+    h.ch = ch;
+    if (mode == CTL_DT_ONLY) {
+        h.dt = dt;
+        return;
+    }
+    // device loop: solver.py:475-479
+    if (!(h.t < h.t_end && h.steps < h.max_steps)) { h.done = 1; return; }
+    const double rem = h.t_end - h.t;
+    if (rem < dt) dt = rem;
+    if (dt <= 1e-16) { h.done = 1; return; }
+    h.dt = dt;
+    h.decay = ch > 0.0 ? exp(-h.glm_alpha * ch * dt / h.min_dxdy) : 1.0;
+    h.pending = 1;
+}
+
+
 __device__ __forceinline__ void step_ctl_block(StepCtl* ctl, const double* part_dt, int n_part, const double* part_bf,
                                                int n_bf, double dx, double dy, double host_dt, double cfl, int mode) {
     __shared__ double sh[NPART_DT][4];
     __shared__ double shb[2][4];
+    // thread 0 works on a register copy of the control block, loaded here so
+    // that its latency overlaps the partials' and stored back once
+    StepCtl h;
+    if (threadIdx.x == 0) h = *ctl;
+    // Eight partials per thread in flight at once (one load latency for 2048
+    // partials with 256 threads); the maxima are exact in any order, and
+    // every partial is >= 0, so a missing one counts as 0.
+    constexpr int U = 8;
     double m[NPART_DT] = {0.0, 0.0, 0.0, 0.0};
-    for (int k = threadIdx.x; k < n_part; k += blockDim.x) {
+    for (int k0 = threadIdx.x; k0 < n_part; k0 += blockDim.x * U) {
+        double v[U][NPART_DT];
 #pragma unroll
-        for (int q = 0; q < NPART_DT; ++q) m[q] = fmax(m[q], part_dt[(long)k * NPART_DT + q]);
+        for (int u = 0; u < U; ++u) {
+            const int k = k0 + u * (int)blockDim.x;
+#pragma unroll
+            for (int q = 0; q < NPART_DT; ++q) v[u][q] = k < n_part ? part_dt[(long)k * NPART_DT + q] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+#pragma unroll
+            for (int q = 0; q < NPART_DT; ++q) m[q] = fmax(m[q], v[u][q]);
+        }
     }
     // boundary flux: strips summed in a fixed order by one wavefront
     double b0 = 0.0, b1 = 0.0;
@@ -689,44 +751,8 @@ __device__ __forceinline__ void step_ctl_block(StepCtl* ctl, const double* part_
     }
     const double bflux = part_bf ? (shb[0][0] - shb[1][0]) : 0.0;
 
-    if (mode == CTL_HOST_STEP_END) {
-        ctl->bflux = bflux;
-        ctl->bflux_cum += bflux * host_dt;
-        return;
-    }
-    if (mode == CTL_DEVICE_LOOP && ctl->pending) {
-        ctl->bflux = bflux;
-        ctl->bflux_cum += bflux * ctl->dt;
-        ctl->t += ctl->dt;
-        ctl->steps += 1;
-        ctl->cur = (ctl->cur + 1) % 3;
-        ctl->pending = 0;
-    }
-    if (mode == CTL_DEVICE_LOOP) {
-        if (ctl->done) return;
-        cfl = ctl->cfl;
-    }
-
-    const double cfmax = sqrt(r[0]);   // partials hold max cf^2
-    const double ch = fmax(cfmax + r[1], 1.0) * 1.5;
-    const double sx = r[2] + ch, sy = r[3] + ch;
-    const double sm = fmax(sx / dx, sy / dy);
-    double dt = (sm < 1e-14) ? cfl * fmin(dx, dy) : cfl / sm;
-    ctl->cfmax = cfmax;
-    ctl->vmax = r[1];
-    ctl->ch = ch;
-    if (mode == CTL_DT_ONLY) {
-        ctl->dt = dt;
-        return;
-    }
-    // device loop: solver.py:475-479
-    if (!(ctl->t < ctl->t_end && ctl->steps < ctl->max_steps)) { ctl->done = 1; return; }
-    const double rem = ctl->t_end - ctl->t;
-    if (rem < dt) dt = rem;
-    if (dt <= 1e-16) { ctl->done = 1; return; }
-    ctl->dt = dt;
-    ctl->decay = ch > 0.0 ? exp(-ctl->glm_alpha * ch * dt / ctl->min_dxdy) : 1.0;
-    ctl->pending = 1;
+    step_ctl_update(h, r, bflux, dx, dy, host_dt, cfl, mode);
+    *ctl = h;
 }
 
 __global__ __launch_bounds__(256) void step_ctl_kernel(StepCtl* ctl, const double* part_dt, int n_part,
