@@ -527,19 +527,19 @@ __global__ __launch_bounds__(256) void closure_source_kernel(const float* __rest
     Fluxes F;
     limited_fluxes(fc, fs, face_weights(theta, k, i, j, nx, ny), face_weights(theta_mix, k, i, j, nx, ny), q, F);
     real* s = (real*)outs.p[m];
-    bool named[10] = {false, false, false, false, false, false, false, false, false, false};
+    unsigned named = 0u;   // variables written, one bit each (a register, not a private array)
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
         const double v = -((F.xr[f] - F.xl[f]) * rdx + (F.yr[f] - F.yl[f]) * rdy);
         real& o = s[fs.field[f] * plane + c];
         if (ACC) o = real(double(o) + sd.dt[m] * v);
         else o = real(v);
-        named[fs.field[f]] = true;
+        named |= 1u << fs.field[f];
     }
     if (!ACC) {
 #pragma unroll
         for (int v = 0; v < 10; ++v)
-            if (!named[v]) s[v * plane + c] = real(0);
+            if (!((named >> v) & 1u)) s[v * plane + c] = real(0);
     }
 }
 
