@@ -80,6 +80,34 @@ __device__ __forceinline__ Frag gather_bf16(const u16* __restrict__ x, long p, i
     return f;
 }
 
+// The same fragment for a pixel whose (image, row, column) the caller has
+// already split off p once (the forward pass gathers 9 taps and every K
+// chunk of one pixel, so the two 64-bit divisions leave the K loop).
+struct PixelPos {
+    int nimg, i, j;
+};
+__device__ __forceinline__ PixelPos pixel_pos(long p, int nx, int ny) {
+    const long plane = (long)nx * ny;
+    PixelPos q;
+    q.nimg = (int)(p / plane);
+    const long r = p - (long)q.nimg * plane;
+    q.i = (int)(r / ny);
+    q.j = (int)(r - (long)q.i * ny);
+    return q;
+}
+template <int C>
+__device__ __forceinline__ Frag gather_bf16_at(const u16* __restrict__ x, bool valid, const PixelPos& q, int k0,
+                                               int sign, int nx, int ny) {
+    Frag f;
+    f.q = make_uint4(0, 0, 0, 0);
+    const int tap = k0 / C, c0 = k0 % C;
+    if (!valid || tap >= 9) return f;
+    const long s = src_pixel(q.nimg, q.i, q.j, sign * (tap / 3 - 1), sign * (tap % 3 - 1), nx, ny);
+    if (s < 0) return f;
+    f.q = *(const uint4*)(x + s * C + c0);
+    return f;
+}
+
 // FP32 source converted to BF16 on load.
 template <int C>
 __device__ __forceinline__ Frag gather_f32(const float* __restrict__ x, long p, int k0, int sign, int nx, int ny) {
@@ -118,9 +146,10 @@ __global__ __launch_bounds__(256) void conv_fwd_kernel(const u16* __restrict__ x
     f32x4 acc[NT];
 #pragma unroll
     for (int t = 0; t < NT; ++t) acc[t] = (f32x4){0.f, 0.f, 0.f, 0.f};
+    const PixelPos pos = pixel_pos(pa >= 0 ? pa : 0, nx, ny);
     for (int kc = 0; kc < KPAD; kc += 32) {
         const int k0 = kc + 8 * q;
-        const Frag a = gather_bf16<CIN>(x, pa, k0, +1, nx, ny);
+        const Frag a = gather_bf16_at<CIN>(x, pa >= 0, pos, k0, +1, nx, ny);
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
             Frag b;
