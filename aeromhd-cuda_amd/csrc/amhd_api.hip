@@ -968,6 +968,8 @@ struct Batch {
     std::vector<int> part_cap;
     BatchInst* dinst = nullptr;
     StepCtl* ctl = nullptr;            // [n]
+    std::vector<StepCtl> hctl;         // host copy of ctl from the last read
+    bool hctl_valid = false;           // false from a run's first launch to its final read
     double* staging = nullptr;
     size_t staging_bytes = 0;
     int rows = 0, n_tasks = 0, resident = 0;
@@ -1014,10 +1016,20 @@ int batch_layout(Batch* b, int rows) {
     return 0;
 }
 
+// The control blocks as the device holds them.  Only amhd_batch_run's
+// launches write them, and every entry point returns with the batch's
+// stream drained, so between runs the copy from the last read is exact and
+// no transfer (or wait) is needed.
 int batch_read_ctl(Batch* b, std::vector<StepCtl>& h) {
+    if (b->hctl_valid) {
+        h = b->hctl;
+        return 0;
+    }
     h.resize(b->n);
     HIP_TRY(hipMemcpyAsync(h.data(), b->ctl, sizeof(StepCtl) * b->n, hipMemcpyDeviceToHost, b->stream));
     HIP_TRY(hipStreamSynchronize(b->stream));
+    b->hctl = h;
+    b->hctl_valid = true;
     return 0;
 }
 
@@ -1226,6 +1238,7 @@ int amhd_batch_run(void* handle, int n_steps, const double* cfl, const double* t
         }
         h[k].n_part = b->part_from_stage3[k] ? d.n_strips * d.n_segs : b->n_part[k];
     }
+    b->hctl_valid = false;   // the launches below change the control blocks
     HIP_TRY(hipMemcpyAsync(b->ctl, h.data(), sizeof(StepCtl) * b->n, hipMemcpyHostToDevice, b->stream));
     batch_ctl_kernel<<<b->n, 256, 0, b->stream>>>(b->dinst, 1);
     HIP_TRY(hipGetLastError());
