@@ -245,7 +245,8 @@ int amhd_mfma_probe(const float* A, const float* B, float* D, void* stream);
  * the 9 network fields; mean, std, fscale: host float[9].
  * amhd_closure_features: x (n, nx, ny, 16) BF16, channel f < 9 =
  * (U[fields[f]] - mean[f]) / std[f] in FP32, channels 9..15 zero.
- * amhd_closure_source: from the network output dF (n, nx, ny, 32) FP32 --
+ * amhd_closure_source: from the network output dF (n, nx, ny, 32) FP32,
+ * 16-byte aligned --
  * channel f the x-face flux correction at i-1/2, 9 + f the y-face one at
  * j-1/2, both in units of fscale[f]; fields[0] must be rho (0) and
  * fields[8] rhoC (9), whose channels carry the mixing flux G -- the
