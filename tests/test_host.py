@@ -34,6 +34,33 @@ def test_library_exports_every_declared_symbol():
     assert lib.amhd_abi_version() == 1
 
 
+def test_closure_source_checks_arguments_before_device_work():
+    """amhd_closure_source refuses a network output that is not 16-byte
+    aligned (its kernels read 4 channels per load), and a field order without
+    rho first and rhoC last, before touching any memory; no device needed."""
+    import numpy as np
+    import amhd_lib
+
+    lib = amhd_lib.load()
+    fields = (C.c_int * 9)(0, 1, 2, 3, 4, 5, 6, 7, 9)
+    fs = (C.c_float * 9)(*([1.0] * 9))
+    dt = (C.c_double * 1)(1e-3)
+    buf = np.zeros(64, dtype=np.float32)
+    base = buf.ctypes.data + (-buf.ctypes.data) % 16
+    P = C.c_void_p * 1
+    st, out = P(base), P(base)
+
+    def call(dF, flds):
+        return lib.amhd_closure_source(C.c_void_p(dF), st, out, amhd_lib.AMHD_F32, flds, 1, 4, 4, fs, 0.1, 0.1, dt,
+                                       1.4, 0.5, None, None, 0, None)
+
+    assert call(base + 4, fields) == -1
+    assert b"16-byte aligned" in lib.amhd_last_error()
+    bad = (C.c_int * 9)(1, 0, 2, 3, 4, 5, 6, 7, 9)
+    assert call(base, bad) == -1
+    assert b"rho" in lib.amhd_last_error()
+
+
 def test_params_struct_layout_matches_header():
     import amhd_lib
 
