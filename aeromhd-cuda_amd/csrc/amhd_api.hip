@@ -1216,6 +1216,19 @@ int amhd_batch_run(void* handle, int n_steps, const double* cfl, const double* t
     int rc = batch_read_ctl(b, h);
     if (rc) return rc;
     std::vector<int> start(b->n);
+    // CFL partials of the instances whose stage-3 partials are not current,
+    // DT_LIST instances per launch
+    DtList list{};
+    int cnt = 0, gmax = 0;
+    auto flush = [&]() -> int {
+        if (cnt == 0) return 0;
+        const hipError_t e = b->dtype == AMHD_F64 ? launch_batch_dt_partials_f64(b->dinst, list, cnt, gmax, b->stream)
+                                                  : launch_batch_dt_partials_f32(b->dinst, list, cnt, gmax, b->stream);
+        if (e != hipSuccess) return fail(-2, "amhd_batch_run: CFL partials launch failed: %s", hipGetErrorString(e));
+        cnt = 0;
+        gmax = 0;
+        return 0;
+    };
     for (int k = 0; k < b->n; ++k) {
         const BatchInst& d = b->inst[k];
         start[k] = step_inout[k];
@@ -1226,18 +1239,20 @@ int amhd_batch_run(void* handle, int n_steps, const double* cfl, const double* t
         h[k].min_dxdy = std::min(b->p[k].dx, b->p[k].dy);
         h[k].done = 0; h[k].pending = 0;
         if (!b->part_valid[k]) {
-            const int g = b->dtype == AMHD_F64
-                              ? launch_dt_partials_f64((const double*)d.buf[h[k].cur], d.plane, d.pitch, d.nx, d.ny,
-                                                       d.gamma, d.part_dt, b->part_cap[k], b->stream)
-                              : launch_dt_partials_f32((const float*)d.buf[h[k].cur], d.plane, d.pitch, d.nx, d.ny,
-                                                       d.gamma, d.part_dt, b->part_cap[k], b->stream);
-            if (g < 0) return fail(-2, "amhd_batch_run: CFL partials launch failed");
+            // as many workgroups as a launch for this instance alone
+            const int g = grid_for((long)d.nx * d.ny, 256, b->part_cap[k]);
+            list.k[cnt] = k;
+            list.cur[cnt] = h[k].cur;
+            list.g[cnt] = g;
+            gmax = std::max(gmax, g);
+            if (++cnt == DT_LIST && (rc = flush())) return rc;
             b->n_part[k] = g;
             b->part_valid[k] = 1;
             b->part_from_stage3[k] = 0;
         }
         h[k].n_part = b->part_from_stage3[k] ? d.n_strips * d.n_segs : b->n_part[k];
     }
+    if ((rc = flush())) return rc;
     b->hctl_valid = false;   // the launches below change the control blocks
     HIP_TRY(hipMemcpyAsync(b->ctl, h.data(), sizeof(StepCtl) * b->n, hipMemcpyHostToDevice, b->stream));
     batch_ctl_kernel<<<b->n, 256, 0, b->stream>>>(b->dinst, 1);

@@ -26,6 +26,11 @@ int launch_dt_partials_f32(const float* u, long plane, int pitch, int nx, int ny
     return launch_dt_partials_impl<float>(u, plane, pitch, nx, ny, gamma, part, max_parts, s);
 }
 
+hipError_t launch_batch_dt_partials_f32(const BatchInst* insts, const DtList& list, int cnt, int gmax,
+                                        hipStream_t s) {
+    return launch_batch_dt_partials_impl<float>(insts, list, cnt, gmax, s);
+}
+
 hipError_t launch_ingest_f32(const double* ug, float* u, long plane, int pitch, int nx, int ny, hipStream_t s) {
     return launch_ingest_impl<float>(ug, u, plane, pitch, nx, ny, s);
 }

@@ -25,6 +25,11 @@ int launch_dt_partials_f64(const double* u, long plane, int pitch, int nx, int n
     return launch_dt_partials_impl<double>(u, plane, pitch, nx, ny, gamma, part, max_parts, s);
 }
 
+hipError_t launch_batch_dt_partials_f64(const BatchInst* insts, const DtList& list, int cnt, int gmax,
+                                        hipStream_t s) {
+    return launch_batch_dt_partials_impl<double>(insts, list, cnt, gmax, s);
+}
+
 hipError_t launch_ingest_f64(const double* ug, double* u, long plane, int pitch, int nx, int ny, hipStream_t s) {
     return launch_ingest_impl<double>(ug, u, plane, pitch, nx, ny, s);
 }

@@ -588,12 +588,14 @@ __global__ __launch_bounds__(WAVE* WPB) AMHD_STAGE_OCC void stage_kernel(StagePa
 // ------------------------------------------------- CFL partials from state
 // One partial per workgroup over the interior layout (used when no stage-3
 // partials are current, e.g. right after a state upload).
+// Workgroup blockIdx.x of `nblocks` covers cells blockIdx.x * 256 + t,
+// stepping by nblocks * 256.
 template <class real>
-__global__ __launch_bounds__(256) void dt_partials_kernel(const real* __restrict__ u, long plane, int pitch,
-                                                          int nx, int ny, double gamma, double* part) {
+__device__ __forceinline__ void dt_partials_block(const real* __restrict__ u, long plane, int pitch, int nx, int ny,
+                                                  double gamma, double* part, int nblocks) {
     real m[NPART_DT] = {0, 0, 0, 0};
     const long n = (long)nx * ny;
-    for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x) {
+    for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n; k += (long)nblocks * blockDim.x) {
         const long i = k / ny, jj = k - i * ny;
         real c[NV];
         load_cell(u + i * pitch + jj, plane, c);
@@ -612,6 +614,12 @@ __global__ __launch_bounds__(256) void dt_partials_kernel(const real* __restrict
         for (int q = 1; q < (int)(blockDim.x >> 6); ++q) r = fmax(r, sh[threadIdx.x][q]);
         part[(long)blockIdx.x * NPART_DT + threadIdx.x] = r;
     }
+}
+
+template <class real>
+__global__ __launch_bounds__(256) void dt_partials_kernel(const real* __restrict__ u, long plane, int pitch,
+                                                          int nx, int ny, double gamma, double* part) {
+    dt_partials_block(u, plane, pitch, nx, ny, gamma, part, (int)gridDim.x);
 }
 
 // ------------------------------------------------------------- step control
@@ -994,6 +1002,24 @@ struct BatchInst {
     int n_strips, n_segs, task_base, pad;
     double gamma, dx, dy;
 };
+
+// CFL partials of up to DT_LIST batch instances in one launch: row y of
+// the grid is instance k[y], its state in buffer cur[y], over g[y]
+// workgroups -- the cells and partials that dt_partials_kernel with g[y]
+// workgroups gives that instance alone.
+constexpr int DT_LIST = 32;
+struct DtList {
+    int k[DT_LIST], cur[DT_LIST], g[DT_LIST];
+};
+
+template <class real>
+__global__ __launch_bounds__(256) void batch_dt_partials_kernel(const BatchInst* __restrict__ insts, DtList list) {
+    const int y = blockIdx.y;
+    const int g = list.g[y];
+    if ((int)blockIdx.x >= g) return;   // uniform over the workgroup
+    const BatchInst& d = insts[list.k[y]];
+    dt_partials_block((const real*)d.buf[list.cur[y]], d.plane, d.pitch, d.nx, d.ny, d.gamma, d.part_dt, g);
+}
 
 template <class real, class P, int STAGE>
 __global__ __launch_bounds__(WAVE* WPB) AMHD_STAGE_OCC void batch_stage_kernel(const BatchInst* __restrict__ insts, int n_inst,

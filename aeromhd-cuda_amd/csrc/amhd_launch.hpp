@@ -32,6 +32,10 @@ int launch_dt_partials_f64(const double* u, long plane, int pitch, int nx, int n
                            int max_parts, hipStream_t s);
 int launch_dt_partials_f32(const float* u, long plane, int pitch, int nx, int ny, double gamma, double* part,
                            int max_parts, hipStream_t s);
+// The same for list entries 0 .. cnt-1 of batch instances (gmax = the
+// largest list.g).
+hipError_t launch_batch_dt_partials_f64(const BatchInst* insts, const DtList& list, int cnt, int gmax, hipStream_t s);
+hipError_t launch_batch_dt_partials_f32(const BatchInst* insts, const DtList& list, int cnt, int gmax, hipStream_t s);
 
 hipError_t launch_ingest_f64(const double* ug, double* u, long plane, int pitch, int nx, int ny, hipStream_t s);
 hipError_t launch_ingest_f32(const double* ug, float* u, long plane, int pitch, int nx, int ny, hipStream_t s);
@@ -124,6 +128,13 @@ int launch_dt_partials_impl(const real* u, long plane, int pitch, int nx, int ny
     const int g = grid_for((long)nx * ny, 256, max_parts);
     dt_partials_kernel<real><<<g, 256, 0, s>>>(u, plane, pitch, nx, ny, gamma, part);
     return hipGetLastError() == hipSuccess ? g : -1;
+}
+
+template <class real>
+hipError_t launch_batch_dt_partials_impl(const BatchInst* insts, const DtList& list, int cnt, int gmax,
+                                         hipStream_t s) {
+    batch_dt_partials_kernel<real><<<dim3(gmax, cnt), 256, 0, s>>>(insts, list);
+    return hipGetLastError();
 }
 
 template <class real>
