@@ -532,32 +532,64 @@ class _Corrector:
         self.src_ptrs = [self.src.data_ptr() + m * step for m in range(n)]
         torch.cuda.synchronize()
 
+    def _plan(self):
+        """The ctypes arguments that stay the same from step to step (buffers,
+        weights, scales, stream), built once and again only when the model's
+        BF16 weights or its statistics are replaced: a step then makes only
+        the calls themselves."""
+        m = self.model
+        if getattr(self, "_plan_wb", None) is m.Wb and self._plan_stats is m.stats:
+            return
+        cfg, n = self.cfg, self.n
+        conv, h = [], self.x
+        for li, (cin, cout, relu) in enumerate(LAYERS):
+            y = self.outs[li]
+            conv.append((_p(h), n, cfg.nx, cfg.ny, cin, _p(m.Wb[li]), _p(m.b[li]), cout, 1 if relu else 0, _p(y),
+                         1 if li == len(LAYERS) - 1 else 0, self.stream))
+            h = y
+        self._conv_args = conv
+        self._feat_args = (_int9(FEAT), n, cfg.nx, cfg.ny, _f9(m.stats["in_mean"]), _f9(m.stats["in_std"]),
+                           _p(self.x), self.stream)
+        self._fields = _int9(FEAT)
+        self._fscale = _f9(m.stats["f_scale"] * self.mask)
+        self._srcs = (C.c_void_p * n)(*self.src_ptrs)
+        self._plan_wb, self._plan_stats = m.Wb, m.stats
+
     def predict(self, pre):
-        cfg = self.cfg
-        x = features(list(pre), self.dtype, FEAT, cfg.nx, cfg.ny, self.model.stats, self.x, self.stream)
-        self.model.forward(x, stream=self.stream, bufs=self.outs)
+        """Features and the four convolutions of the network pass, on the
+        stream; the same launches as features() and FluxCNN.forward."""
+        if len(pre) != self.n:
+            raise ValueError(f"predict: {len(pre)} states for a corrector of {self.n}")
+        self._plan()
+        lib = self.model.lib
+        _L.check(lib.amhd_closure_features((C.c_void_p * len(pre))(*pre), self.dtype, *self._feat_args),
+                 "amhd_closure_features")
+        for a in self._conv_args:
+            _L.check(lib.amhd_conv3x3_fwd(*a), "amhd_conv3x3_fwd")
 
     def check(self, ptrs, dt) -> np.ndarray:
         """(n, 4): min rho, min p, max|divB|, non-finite cells of U + dt S."""
+        self._plan()
         cfg, n = self.cfg, len(ptrs)
         out = np.empty((n, 4))
-        V = C.c_void_p * n
-        _L.check(_L.load().amhd_closure_check(V(*ptrs), V(*self.src_ptrs[:n]), _dvec(dt), self.dtype, n, cfg.nx,
-                                              cfg.ny, float(cfg.gamma), float(cfg.dx), float(cfg.dy),
-                                              _p(self.work), out.ctypes.data_as(C.POINTER(C.c_double)),
-                                              self.stream), "amhd_closure_check")
+        srcs = self._srcs if n == self.n else (C.c_void_p * n)(*self.src_ptrs[:n])
+        _L.check(self.model.lib.amhd_closure_check((C.c_void_p * n)(*ptrs), srcs, _dvec(dt), self.dtype, n, cfg.nx,
+                                                   cfg.ny, float(cfg.gamma), float(cfg.dx), float(cfg.dy),
+                                                   _p(self.work), out.ctypes.data_as(C.POINTER(C.c_double)),
+                                                   self.stream), "amhd_closure_check")
         return out
 
     def correct(self, pre, post, dt, div0):
         """Source against the post-step states (bounds also from the pre-step
         ones), guard, apply; returns (accepted mask, |divB| of the states now
         current)."""
+        self._plan()
         cfg, n = self.cfg, self.n
-        lib = _L.load()
+        lib = self.model.lib
         V = C.c_void_p * n
         lim = self.keep is not None
-        _L.check(lib.amhd_closure_source(_p(self.outs[-1]), V(*post), V(*self.src_ptrs), self.dtype, _int9(FEAT), n,
-                                         cfg.nx, cfg.ny, _f9(self.model.stats["f_scale"] * self.mask),
+        _L.check(lib.amhd_closure_source(_p(self.outs[-1]), V(*post), self._srcs, self.dtype, self._fields, n,
+                                         cfg.nx, cfg.ny, self._fscale,
                                          float(cfg.dx), float(cfg.dy), _dvec(dt), float(cfg.gamma),
                                          float(self.keep) if lim else 0.0, _p(self.theta) if lim else None,
                                          V(*pre) if lim else None, 0, self.stream), "amhd_closure_source")
