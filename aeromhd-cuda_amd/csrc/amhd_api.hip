@@ -968,6 +968,7 @@ struct Batch {
     std::vector<int> part_cap;
     BatchInst* dinst = nullptr;
     StepCtl* ctl = nullptr;            // [n]
+    StepCtl* ctl_pin = nullptr;        // [n] pinned host staging of ctl's transfers
     std::vector<StepCtl> hctl;         // host copy of ctl from the last read
     bool hctl_valid = false;           // false from a run's first launch to its final read
     double* staging = nullptr;
@@ -1025,9 +1026,9 @@ int batch_read_ctl(Batch* b, std::vector<StepCtl>& h) {
         h = b->hctl;
         return 0;
     }
-    h.resize(b->n);
-    HIP_TRY(hipMemcpyAsync(h.data(), b->ctl, sizeof(StepCtl) * b->n, hipMemcpyDeviceToHost, b->stream));
+    HIP_TRY(hipMemcpyAsync(b->ctl_pin, b->ctl, sizeof(StepCtl) * b->n, hipMemcpyDeviceToHost, b->stream));
     HIP_TRY(hipStreamSynchronize(b->stream));
+    h.assign(b->ctl_pin, b->ctl_pin + b->n);
     b->hctl = h;
     b->hctl_valid = true;
     return 0;
@@ -1059,6 +1060,7 @@ int amhd_batch_destroy(void* handle) {
     }
     if (b->dinst) hipFree(b->dinst);
     if (b->ctl) hipFree(b->ctl);
+    if (b->ctl_pin) hipHostFree(b->ctl_pin);
     if (b->staging) hipFree(b->staging);
     if (b->stream) hipStreamDestroy(b->stream);
     delete b;
@@ -1106,6 +1108,7 @@ int amhd_batch_create(void** out, int n, const int* nx, const int* ny, int dtype
     }
     if (e == hipSuccess) e = hipMalloc(&b->staging, b->staging_bytes);
     if (e == hipSuccess) e = hipMalloc(&b->ctl, sizeof(StepCtl) * n);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&b->ctl_pin, sizeof(StepCtl) * n, hipHostMallocDefault);
     if (e == hipSuccess) e = hipMalloc(&b->dinst, sizeof(BatchInst) * n);
     if (e == hipSuccess) {
         std::vector<StepCtl> z(n);
@@ -1254,7 +1257,10 @@ int amhd_batch_run(void* handle, int n_steps, const double* cfl, const double* t
     }
     if ((rc = flush())) return rc;
     b->hctl_valid = false;   // the launches below change the control blocks
-    HIP_TRY(hipMemcpyAsync(b->ctl, h.data(), sizeof(StepCtl) * b->n, hipMemcpyHostToDevice, b->stream));
+    // through the pinned staging, whose last use (a read) has completed; the
+    // final read below reuses it after this copy in stream order
+    std::memcpy(b->ctl_pin, h.data(), sizeof(StepCtl) * b->n);
+    HIP_TRY(hipMemcpyAsync(b->ctl, b->ctl_pin, sizeof(StepCtl) * b->n, hipMemcpyHostToDevice, b->stream));
     batch_ctl_kernel<<<b->n, 256, 0, b->stream>>>(b->dinst, 1);
     HIP_TRY(hipGetLastError());
     if (use_graph && n_steps > 0) {
