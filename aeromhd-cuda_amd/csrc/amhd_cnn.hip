@@ -24,6 +24,7 @@
 #include <string>
 #include <vector>
 #include <cmath>
+#include <mutex>
 
 #include "aeromhd.h"
 
@@ -861,7 +862,17 @@ int amhd_closure_check(const void* const* states, const void* const* srcs, const
         return cfail(-1, "amhd_closure_check: bad argument");
     if (dtype != AMHD_F64 && dtype != AMHD_F32) return cfail(-1, "amhd_closure_check: bad dtype %d", dtype);
     hipStream_t s = (hipStream_t)stream;
-    std::vector<double> h((size_t)CHUNK * CHECK_BLOCKS * 4);
+    // The per-workgroup results come back through one pinned host buffer of
+    // the process (64 KiB, allocated at the first call and kept), one call
+    // at a time.
+    static std::mutex pin_mu;
+    static double* pin = nullptr;
+    std::lock_guard<std::mutex> lock(pin_mu);
+    if (!pin && hipHostMalloc((void**)&pin, sizeof(double) * CHUNK * CHECK_BLOCKS * 4, hipHostMallocPortable) != hipSuccess) {
+        pin = nullptr;
+        return cfail(-3, "amhd_closure_check: pinned staging allocation failed");
+    }
+    const double* h = pin;
     for (int m0 = 0; m0 < n; m0 += CHUNK) {
         const int cnt = n - m0 < CHUNK ? n - m0 : CHUNK;
         ConstPtrs st{}, sr{};
@@ -877,12 +888,12 @@ int amhd_closure_check(const void* const* states, const void* const* srcs, const
         else closure_check_kernel<float><<<grid, 256, 0, s>>>(st, sr, sd, nx, ny, gamma, dx, dy, work);
         if (hipGetLastError() != hipSuccess) return cfail(-2, "amhd_closure_check: launch failed");
         const size_t bytes = sizeof(double) * (size_t)cnt * CHECK_BLOCKS * 4;
-        if (hipMemcpyAsync(h.data(), work, bytes, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        if (hipMemcpyAsync(pin, work, bytes, hipMemcpyDeviceToHost, s) != hipSuccess ||
             hipStreamSynchronize(s) != hipSuccess)
             return cfail(-2, "amhd_closure_check: copy failed");
         for (int m = 0; m < cnt; ++m) {
             double* o = out + 4 * (size_t)(m0 + m);
-            const double* p = h.data() + (size_t)m * CHECK_BLOCKS * 4;
+            const double* p = h + (size_t)m * CHECK_BLOCKS * 4;
             o[0] = p[0]; o[1] = p[1]; o[2] = p[2]; o[3] = p[3];
             for (int b = 1; b < CHECK_BLOCKS; ++b) {
                 o[0] = std::fmin(o[0], p[4 * b]); o[1] = std::fmin(o[1], p[4 * b + 1]);
