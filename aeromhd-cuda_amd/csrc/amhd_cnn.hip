@@ -602,12 +602,20 @@ __global__ __launch_bounds__(256) void closure_source_kernel(const float* __rest
     }
 }
 
+// Workgroups per instance (the partials the caller's workspace holds) and
+// their size: 512 lanes give a 256x128 instance one cell per lane, and
+// twice the wavefronts in flight of 256-lane workgroups for this
+// latency-bound pass.  The results are minima, maxima and a count, the same
+// for any split of the cells.
+constexpr int CHECK_BLOCKS = 64;
+constexpr int CHECK_THREADS = 512;
+
 // Guard quantities of U + dt S without writing it: per instance (blockIdx.y)
 // and workgroup, {min rho, min p, max |divB|, non-finite cells} with
 // p = (gamma-1)(E - |m|^2/(2 rho) - |B|^2/2) and divB as amhd_state_check
 // (central differences on rows and columns 1 .. n-2), all in FP64.
 template <class real>
-__global__ __launch_bounds__(256) void closure_check_kernel(ConstPtrs st, ConstPtrs src, StepDt sd, int nx, int ny,
+__global__ __launch_bounds__(CHECK_THREADS) void closure_check_kernel(ConstPtrs st, ConstPtrs src, StepDt sd, int nx, int ny,
                                                             double gamma, double dx, double dy,
                                                             double* __restrict__ part) {
     const int m = blockIdx.y;
@@ -637,7 +645,7 @@ __global__ __launch_bounds__(256) void closure_check_kernel(ConstPtrs st, ConstP
             md = fmax(md, fabs(d));
         }
     }
-    __shared__ double sh[4][4];
+    __shared__ double sh[4][CHECK_THREADS / 64];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     for (int off = 32; off > 0; off >>= 1) {
         mr = fmin(mr, __shfl_xor(mr, off, 64));
@@ -670,7 +678,7 @@ __global__ __launch_bounds__(256) void closure_apply_kernel(MutPtrs st, ConstPtr
     for (int v = 0; v < 10; ++v) u[v * plane + c] = real(double(u[v * plane + c]) + sd.dt[m] * double(s[v * plane + c]));
 }
 
-constexpr int CHECK_BLOCKS = 64;   // workgroups per instance in closure_check_kernel
+
 
 }  // namespace
 
@@ -884,8 +892,9 @@ int amhd_closure_check(const void* const* states, const void* const* srcs, const
             sd.dt[m] = dt[m0 + m];
         }
         const dim3 grid(CHECK_BLOCKS, cnt);
-        if (dtype == AMHD_F64) closure_check_kernel<double><<<grid, 256, 0, s>>>(st, sr, sd, nx, ny, gamma, dx, dy, work);
-        else closure_check_kernel<float><<<grid, 256, 0, s>>>(st, sr, sd, nx, ny, gamma, dx, dy, work);
+        if (dtype == AMHD_F64)
+            closure_check_kernel<double><<<grid, CHECK_THREADS, 0, s>>>(st, sr, sd, nx, ny, gamma, dx, dy, work);
+        else closure_check_kernel<float><<<grid, CHECK_THREADS, 0, s>>>(st, sr, sd, nx, ny, gamma, dx, dy, work);
         if (hipGetLastError() != hipSuccess) return cfail(-2, "amhd_closure_check: launch failed");
         const size_t bytes = sizeof(double) * (size_t)cnt * CHECK_BLOCKS * 4;
         if (hipMemcpyAsync(pin, work, bytes, hipMemcpyDeviceToHost, s) != hipSuccess ||
