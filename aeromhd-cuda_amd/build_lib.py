@@ -85,7 +85,8 @@ def build(out_dir: str | None = None, verbose: bool = False, extra_flags=()) -> 
 if __name__ == "__main__":
     # python build_lib.py [-v] [--wave-times OUT_DIR]: the regular library, or
     # a diagnostic one that records every stage-kernel wavefront's start and
-    # end time (exported as amhd_debug_wave_times; see wave_times.py)
+    # end time (exported as amhd_debug_wave_times: [stage][wave][start, end] in
+    # ticks of the 100 MHz s_memrealtime clock)
     if "--wave-times" in sys.argv:
         out = sys.argv[sys.argv.index("--wave-times") + 1]
         print(build(out, verbose="-v" in sys.argv, extra_flags=("-DAMHD_WAVE_TIMES",)))

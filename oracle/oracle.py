@@ -27,9 +27,28 @@ tests/ named in brackets):
   uniform state preserved, x<->y transpose symmetry, and the reference's
   published V&V: Brio-Wu checks (main.py:88-97), Alfven L1(By) errors
   (PDF p.3: ~3.1e-8, 8.7e-9, 2.2e-9, 5.5e-10) and order >= 1.5 (published
-  1.96), contact p and vx constant [test_oracle_vv.py].
-* orc_boundary_flux -- closed form on a uniform flow; parity unpinned beyond
-  that (the reference sums with CuPy's unpublished reduction order).
+  1.96), contact p and vx constant [test_oracle_vv.py].  The stage weights
+  by the scheme's temporal order: on a checkerboard state (every slope 0,
+  so the semi-discrete system is a smooth ODE) the error falls 6-12x per
+  halving of dt, and the same step composed with wrong weights fails
+  [test_oracle_pins_step.py::test_ssprk3_*].  The HIP path reproduces the
+  reference's published RMI results through this step: peak integral
+  mixing widths 0.0968 / 0.0879 / 0.0579 (PDF p.5) and max|divB| peaks
+  ~13 / ~27 (PDF p.4) [test_gpu_workloads.py], and equals the oracle for
+  the first 150 steps of each case.
+* orc_compute_rhs's Powell source (solver.py:242-266) -- R(on) - R(off) =
+  -divB (0, B, v, v.B, 0, 0) on fields with Bx linear in x and By linear
+  in y, whose divergence is known [test_oracle_pins_step.py::test_powell_*].
+* orc_enforce_bounds -- case by case: C in [0, 1] kept, C > 1 -> rho,
+  C < 0 -> 0, rho below the floor -> the floor
+  [test_oracle_pins_step.py::test_scalar_bounds_case_by_case].
+* orc_boundary_flux -- the textbook ideal-MHD energy flux summed over two
+  non-uniform boundary columns (every other column holding unrelated
+  values), and the energy balance of an RMI run: the raw drift is
+  large and the drift corrected by the accumulated flux is ~0, while the
+  opposite sign breaks it (PDF p.1) [test_oracle_pins_step.py::test_boundary_*].
+  The sum order of the reference (CuPy's reduction) is not published, so
+  the flux is compared at 1e-13 relative, not bit for bit.
 """
 from __future__ import annotations
 
