@@ -68,6 +68,7 @@ SIGNATURES = {
     "amhd_apply_bc": (_i, [_vp, _i, _i, _i, _i, _vp]),
     "amhd_enforce_scalar_bounds": (_i, [_vp, _l, _vp]),
     "amhd_compute_dt_w": (_i, [_vp, _l, _d, _d, _d, _d, _dp, _dp, _vp]),
+    "amhd_compute_dt_w_noalloc": (_i, [_vp, _l, _d, _d, _d, _d, _dp, _dp, _vp]),
     "amhd_batch_create": (_i, [C.POINTER(_vp), _i, _ip, _ip, _i, C.POINTER(AmhdParams)]),
     "amhd_batch_destroy": (_i, [_vp]),
     "amhd_batch_set_rows": (_i, [_vp, _i]),

@@ -19,23 +19,23 @@ using namespace amhd;
 
 namespace {
 
-thread_local std::string g_err;
+// The last error's text, per thread, in a fixed buffer (reporting an error
+// allocates nothing).
+thread_local char g_err[512];
 
 int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
 int fail(int code, const char* fmt, ...) {
-    char buf[512];
     va_list ap;
     va_start(ap, fmt);
-    vsnprintf(buf, sizeof buf, fmt, ap);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
     va_end(ap);
-    g_err = buf;
     return code;
 }
 
 }  // namespace
 
 int amhd_set_error(int code, const char* text) {
-    g_err = text;
+    snprintf(g_err, sizeof g_err, "%s", text ? text : "");
     return code;
 }
 
@@ -72,6 +72,37 @@ int get_scratch(Scratch** out) {
         it = g_scratch.emplace(dev, s).first;
     }
     *out = &it->second;
+    return 0;
+}
+
+// Scratch of amhd_compute_dt_w_noalloc: a file-scope device array instead of
+// allocated memory, for callers that cannot allocate.  NOALLOC_PARTS CFL
+// partials (NPART_DT doubles each), then one StepCtl: 32 KiB + 128 B per
+// device, held until the process ends.  In host code `scratch` only names
+// the symbol; its device address comes from hipGetSymbolAddress, once per
+// device (noalloc_scratch), and only that address is passed to kernels.
+constexpr int NOALLOC_PARTS = 1024;
+constexpr int NOALLOC_CTL = NOALLOC_PARTS * NPART_DT;   // offset of the StepCtl, in doubles
+constexpr int NOALLOC_DOUBLES = NOALLOC_CTL + (int)((sizeof(StepCtl) + sizeof(double) - 1) / sizeof(double));
+static_assert(NOALLOC_CTL * sizeof(double) % alignof(StepCtl) == 0, "StepCtl alignment");
+static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_noalloc_mu;
+double* g_noalloc_ptr[NOALLOC_MAX_DEVICES] = {};
+
+int noalloc_scratch(double** out) {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return fail(-4, "amhd_compute_dt_w_noalloc: device %d beyond %d", dev,
+                                                           NOALLOC_MAX_DEVICES);
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (!g_noalloc_ptr[dev]) {
+        void* p = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_noalloc_ptr[dev] = (double*)p;
+    }
+    *out = g_noalloc_ptr[dev];
     return 0;
 }
 
@@ -358,7 +389,7 @@ __global__ __launch_bounds__(256) void conc_profile_kernel(const real* __restric
 
 extern "C" {
 
-const char* amhd_last_error(void) { return g_err.c_str(); }
+const char* amhd_last_error(void) { return g_err; }
 int amhd_abi_version(void) { return AMHD_ABI_VERSION; }
 
 int amhd_create(void** out, int nx, int ny, int dtype, const amhd_params* p) {
@@ -947,6 +978,35 @@ int amhd_compute_dt_w(const double* W, long n, double gamma, double cfl, double 
     HIP_TRY(hipGetLastError());
     StepCtl h;
     HIP_TRY(hipMemcpyAsync(&h, sc->ctl, sizeof h, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    *dt = h.dt;
+    *ch = h.ch;
+    return 0;
+}
+
+// amhd_compute_dt_w with its CFL partials and step-control block in the
+// library's file-scope device array (no allocation).  Calls on one device
+// share that array: overlapping calls on one device race.  Every field the
+// step control reads back is written by this call (the partials by the
+// partial kernel, dt / ch / cf / v maxima by the control kernel), so the
+// previous call's contents never reach the result.
+int amhd_compute_dt_w_noalloc(const double* W, long n, double gamma, double cfl, double dx, double dy, double* dt,
+                              double* ch, void* stream) {
+    if (!W || n <= 0 || !dt || !ch) return fail(-1, "amhd_compute_dt_w_noalloc: bad argument");
+    const int g = grid_for(n, 256, NOALLOC_PARTS);   // grid-stride: any n fits in NOALLOC_PARTS partials
+    if (g < 1 || g > NOALLOC_PARTS)
+        return fail(-1, "amhd_compute_dt_w_noalloc: %d partials exceed the scratch of %d", g, NOALLOC_PARTS);
+    double* sp = nullptr;
+    int rc = noalloc_scratch(&sp);
+    if (rc) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    StepCtl* ctl = (StepCtl*)(sp + NOALLOC_CTL);
+    dt_partials_w_kernel<<<g, 256, 0, s>>>(W, n, gamma, sp);
+    HIP_TRY(hipGetLastError());
+    step_ctl_kernel<<<1, 256, 0, s>>>(ctl, sp, g, nullptr, 0, dx, dy, 0.0, cfl, CTL_DT_ONLY);
+    HIP_TRY(hipGetLastError());
+    StepCtl h;
+    HIP_TRY(hipMemcpyAsync(&h, ctl, sizeof h, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     *dt = h.dt;
     *ch = h.ch;

@@ -188,6 +188,17 @@ int amhd_enforce_scalar_bounds(double* U, long n, void* stream);
 int amhd_compute_dt_w(const double* W, long n, double gamma, double cfl, double dx, double dy, double* dt,
                       double* ch, void* stream);
 
+/* amhd_compute_dt_w without allocation (no counterpart in the reference): for
+ * callers that cannot allocate device memory (code under graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The same results
+ * bit for bit; its CFL partials (at most 1024) and step-control block live
+ * in a file-scope device array of the library, 32 KiB + 128 B per device,
+ * held until the process ends.  The partial grid is capped at 1024
+ * workgroups (grid-stride), so every n >= 1 fits.  Calls on one device share
+ * that array: overlapping calls on one device race.  Synchronises. */
+int amhd_compute_dt_w_noalloc(const double* W, long n, double gamma, double cfl, double dx, double dy, double* dt,
+                              double* ch, void* stream);
+
 /* ---- batched form (no counterpart in the reference): many independent
  * instances of uneven size advanced together, one persistent launch per RK
  * stage over all instances, each with its own device-side dt, glm_ch, psi
