@@ -83,6 +83,9 @@ class SlabSolver:
 
     def _exchange(self, stage: int) -> None:
         n = self.n
+        # every slab's previous stage must be done reading its halo buffers
+        # before the copies below (on the neighbours' streams) refill them
+        self._sync()
         for k in range(n):
             lo, hi = self.halo[k]
             if lo is not None:      # rows i0-2, i0-1: the last two rows of the lower neighbour

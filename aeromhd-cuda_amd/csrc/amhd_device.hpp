@@ -534,6 +534,40 @@ __device__ __forceinline__ real hlld_face(const real (&wl)[NV], const real (&wr)
     return smax;
 }
 
+// The ideal-MHD x-flux of one state with the GLM components (psi, ch^2 Bx):
+// the HLLD flux of a face whose two states are equal (its consistency
+// F(W, W) = F(W); with WL = WR the GLM face values are Bx_f = Bx and
+// psi_f = psi, physics.py:74-82).  FP32 fast path only.
+template <class real>
+__device__ __forceinline__ void phys_flux_x(const real (&w)[NV], real (&f)[NV], const FluxConst<real>& k) {
+    const real rho = w[iRHO], vx = w[iVX], vy = w[iVY], vz = w[iVZ];
+    const real bx = w[iBX], by = w[iBY], bz = w[iBZ], p = w[iPR];
+    const real half = real(0.5);
+    const real b2 = bx * bx + by * by + bz * bz;
+    const real pt = p + half * b2;
+    const real m = rho * vx;
+    const real e = p * k.igm1 + half * (rho * (vx * vx + vy * vy + vz * vz) + b2);
+    f[0] = m;
+    f[1] = m * vx + pt - bx * bx;
+    f[2] = m * vy - bx * by;
+    f[3] = m * vz - bx * bz;
+    f[4] = w[iPSI];
+    f[5] = by * vx - bx * vy;
+    f[6] = bz * vx - bx * vz;
+    f[7] = (e + pt) * vx - bx * (vx * bx + vy * by + vz * bz);
+    f[8] = k.ch2 * bx;
+    f[9] = m * w[iCLR];
+}
+
+// True when every component of d is +0 or -0 (NaN and inf are not).
+template <class real>
+__device__ __forceinline__ bool all_zero(const real (&d)[NV]) {
+    unsigned acc = 0u;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) acc |= __float_as_uint(d[v]);
+    return (acc & 0x7fffffffu) == 0u;
+}
+
 // y-face flux: rotate (vx,vy),(Bx,By), solve, rotate back (physics.py:373-379)
 template <class real, class P, bool GLM = false, bool FLOORED = false>
 __device__ __forceinline__ real hlld_face_y(const real (&wl)[NV], const real (&wr)[NV], real (&f)[NV],

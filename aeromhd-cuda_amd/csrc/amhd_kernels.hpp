@@ -242,6 +242,9 @@ __device__ __forceinline__ void stage_wave(const StageParams<real>& a, const int
         dt_d = a.ctl->dt; ch_d = a.ctl->ch; decay_d = a.ctl->decay;
     }
     const FluxConst<real> kc = make_flux_const<real>(a.gamma, ch_d);
+    // uniform x faces take the physical flux (FP32 with GLM only: the FP64
+    // path keeps the reference's arithmetic bit for bit)
+    constexpr bool XSHORT = !P::exact && GLM;
     const real dtr = real(dt_d);
     const real dxr = real(a.dx), dyr = real(a.dy);
     const real idx = real(1.0 / a.dx), idy = real(1.0 / a.dy);
@@ -316,7 +319,7 @@ __device__ __forceinline__ void stage_wave(const StageParams<real>& a, const int
     real xl[NV], dxp[NV];   // left state of face i+1/2; W_{i+1} - W_i
     real bxm;               // Bx of row i-1 (Powell)
     {
-        real wm2[NV], wm1[NV], fa[NV], fb[NV];
+        real wm2[NV], wm1[NV], fa[NV], fb[NV], dm0[NV];
         prim_row(i0 - 2, wm2);
         prim_row(i0 - 1, wm1);
         load_in(i0, QA);
@@ -325,16 +328,25 @@ __device__ __forceinline__ void stage_wave(const StageParams<real>& a, const int
         cons_to_prim<real, P>(QB, B, kc.gamma, psi_ghost(i0 + 1));
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
-            const real dmm = wm1[v] - wm2[v], dm0 = A[v] - wm1[v];
+            const real dmm = wm1[v] - wm2[v];
+            dm0[v] = A[v] - wm1[v];
             dxp[v] = B[v] - A[v];
-            const real h_m1 = vl_half_d<real, P>(dmm, dm0);
-            const real h_0 = vl_half_d<real, P>(dm0, dxp[v]);
+            const real h_m1 = vl_half_d<real, P>(dmm, dm0[v]);
+            const real h_0 = vl_half_d<real, P>(dm0[v], dxp[v]);
             fa[v] = wm1[v] + h_m1;
             fb[v] = A[v] - h_0;
             xl[v] = A[v] + h_0;
         }
         floor_face<real, P>(fa); floor_face<real, P>(fb); floor_face<real, P>(xl);
         hlld_face<real, P, GLM, true>(fa, fb, F0, kc);
+        if constexpr (XSHORT) {
+            // the uniform-face rule of the row loop, for face i0-1/2
+            if (all_zero(dm0)) {
+                phys_flux_x(A, F0, kc);
+#pragma unroll
+                for (int v = 0; v < NV; ++v) xl[v] = A[v];
+            }
+        }
         bxm = wm1[iBX];
     }
 
@@ -359,7 +371,24 @@ __device__ __forceinline__ void stage_wave(const StageParams<real>& a, const int
         if constexpr (STAGE >= 2) load_row(r_u0, i, ubase);
 
         // ---- x face i+1/2: slope at row i+1 from the differences
-        {
+        // Uniform faces (FP32): where W_i == W_{i+1} in all ten components
+        // (dxp == 0), both van Leer slopes next to the face are 0, so its two
+        // states equal that cell and its flux is the physical flux F(W_{i+1});
+        // the left state of face i+3/2 is W_{i+1}.  Every lane whose face is
+        // uniform gets exactly that, whichever branch its wavefront takes; a
+        // wavefront whose faces are all uniform skips the slopes and the
+        // HLLD solve.
+        bool uni = false;
+        if constexpr (XSHORT) uni = all_zero(dxp);
+        const bool all_uni = XSHORT && __builtin_amdgcn_ballot_w64(!uni) == 0;
+        if (all_uni) {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                dxp[v] = wp2[v] - wp1[v];
+                xl[v] = wp1[v];
+            }
+            phys_flux_x(wp1, fxr, kc);
+        } else {
             real xr[NV], xn[NV];
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
@@ -374,6 +403,13 @@ __device__ __forceinline__ void stage_wave(const StageParams<real>& a, const int
 #pragma unroll
             for (int v = 0; v < NV; ++v) xl[v] = xn[v];
             floor_face<real, P>(xl);
+            if constexpr (XSHORT) {
+                if (uni) {
+                    phys_flux_x(wp1, fxr, kc);
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) xl[v] = wp1[v];
+                }
+            }
         }
 
         // ---- y face j+1/2 of row i (neighbours through DPP lane shifts)
