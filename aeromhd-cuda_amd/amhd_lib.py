@@ -41,6 +41,7 @@ SIGNATURES = {
     "amhd_set_rows": (_i, [_vp, _i]),
     "amhd_get_rows": (_i, [_vp]),
     "amhd_set_packing": (_i, [_vp, _i]),
+    "amhd_set_balance": (_i, [_vp, _i]),
     "amhd_upload_state": (_i, [_vp, _vp, _i]),
     "amhd_download_state": (_i, [_vp, _vp, _i]),
     "amhd_compute_dt": (_i, [_vp, _d, _dp, _dp]),

@@ -140,6 +140,8 @@ struct Ctx {
     hipGraphExec_t graph[3] = {nullptr, nullptr, nullptr};
     const void* halo[2] = {nullptr, nullptr};   // x-slab halo buffers (lo, hi), caller-owned
     int* seg_i0 = nullptr;       // device [nx + 1]: first row of each segment, then nx
+    int balance = 0;             // size the segments by cost each step (FP32; amhd_set_balance)
+    unsigned char* rowflags = nullptr;   // device [nx][RB_FLAG_PITCH]: rows of uniform x faces
 };
 
 
@@ -250,6 +252,8 @@ StageParams<real> stage_args(const Ctx* c, int stage, const real* u0, const real
     a.seg_i0 = c->seg_i0;
     a.pk_groups = c->pk_groups;
     a.pk_shift = c->pk_shift;
+    a.rowflags = (c->balance && stage == 3 && (c->pk_groups ? c->n_strips - 1 : c->n_strips) <= RB_FLAG_PITCH)
+                     ? c->rowflags : nullptr;
     return a;
 }
 
@@ -277,8 +281,17 @@ hipError_t enqueue_step(Ctx* c, int cur, double dt, double ch, double decay, con
 }
 
 hipError_t enqueue_ctl(Ctx* c, int n_part, double host_dt, double cfl, int mode, StepCtl* ctl) {
-    step_ctl_kernel<<<1, 256, 0, c->stream>>>(ctl, c->part_dt, n_part, c->p.bc_x_periodic ? nullptr : c->part_bf,
-                                               c->n_strips, c->p.dx, c->p.dy, host_dt, cfl, mode);
+    const double* bf = c->p.bc_x_periodic ? nullptr : c->part_bf;
+    if (c->balance && mode != CTL_DT_ONLY) {
+        // the step's last stage has recorded its row flags: size the next
+        // step's segments
+        Rebalance rb{c->rowflags, c->pk_groups ? c->n_strips - 1 : c->n_strips, c->nx, c->n_segs, c->seg_i0};
+        step_ctl_balance_kernel<<<1, 256, 0, c->stream>>>(ctl, c->part_dt, n_part, bf, c->n_strips, c->p.dx, c->p.dy,
+                                                          host_dt, cfl, mode, rb);
+    } else {
+        step_ctl_kernel<<<1, 256, 0, c->stream>>>(ctl, c->part_dt, n_part, bf, c->n_strips, c->p.dx, c->p.dy, host_dt,
+                                                  cfl, mode);
+    }
     return hipGetLastError();
 }
 
@@ -435,6 +448,10 @@ int amhd_create(void** out, int nx, int ny, int dtype, const amhd_params* p) {
     if (e == hipSuccess) e = hipMalloc(&c->gsum, sizeof(double) * 4 * GSUM_BLOCKS);
     if (e == hipSuccess) e = hipMalloc(&c->ctl, sizeof(StepCtl));
     if (e == hipSuccess) e = hipMalloc(&c->seg_i0, sizeof(int) * (size_t)(nx + 1));
+    // cost-sized segments where uniform x faces make rows cheaper (FP32)
+    c->balance = dtype == AMHD_F32 ? 1 : 0;
+    if (e == hipSuccess) e = hipMalloc(&c->rowflags, (size_t)RB_FLAG_PITCH * nx);
+    if (e == hipSuccess) e = hipMemsetAsync(c->rowflags, 0, (size_t)RB_FLAG_PITCH * nx, c->stream);
     if (e == hipSuccess) e = upload_segments(c);
     if (e == hipSuccess) {
         StepCtl z{};
@@ -463,6 +480,7 @@ int amhd_destroy(void* ctx) {
     if (c->gsum) hipFree(c->gsum);
     if (c->ctl) hipFree(c->ctl);
     if (c->seg_i0) hipFree(c->seg_i0);
+    if (c->rowflags) hipFree(c->rowflags);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
     return 0;
@@ -477,6 +495,17 @@ int amhd_set_rows(void* ctx, int rows) {
     HIP_TRY(upload_segments(c));
     drop_graphs(c);
     c->part_valid = false;   // per-wave partials follow the tiling
+    return 0;
+}
+
+int amhd_set_balance(void* ctx, int on) {
+    if (!ctx) return fail(-1, "amhd_set_balance: null context");
+    if (on != 0 && on != 1) return fail(-1, "amhd_set_balance: on must be 0 or 1 (got %d)", on);
+    Ctx* c = (Ctx*)ctx;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->balance = on;
+    HIP_TRY(upload_segments(c));
+    drop_graphs(c);
     return 0;
 }
 

@@ -83,7 +83,16 @@ template <class real> struct StageParams {
     const real* halo_hi;
     int first_blocks;       // workgroups dispatched first onto each CU (the CU count); 0 = not known
     const int* seg_i0;      // [n_segs + 1] first row of each segment (null: seg*nx/n_segs)
+    // STAGE 3, wide strips: rowflags[i * RB_FLAG_PITCH + strip] = 1 where row
+    // i's x faces all took the uniform-face path in the strip's wavefront
+    // (null = none); the step control sizes the next step's segments from
+    // them
+    unsigned char* rowflags;
 };
+
+// Row flags of the cost-sized segments: one 16-byte-aligned record per row
+// with a byte per wide strip (at most RB_FLAG_PITCH strips; more: no flags)
+constexpr int RB_FLAG_PITCH = 64;
 
 // Wavefronts of one stage launch: the wide strips' segments, then the
 // packed wavefronts of a narrow last strip.
@@ -410,6 +419,10 @@ __device__ __forceinline__ void stage_wave(const StageParams<real>& a, const int
                     for (int v = 0; v < NV; ++v) xl[v] = wp1[v];
                 }
             }
+        }
+
+        if constexpr (STAGE == 3 && XSHORT && !PK) {
+            if (a.rowflags && lane == 0) a.rowflags[(long)i * RB_FLAG_PITCH + strip] = all_uni ? 1 : 0;
         }
 
         // ---- y face j+1/2 of row i (neighbours through DPP lane shifts)
@@ -802,6 +815,96 @@ __device__ __forceinline__ void step_ctl_block(StepCtl* ctl, const double* part_
 __global__ __launch_bounds__(256) void step_ctl_kernel(StepCtl* ctl, const double* part_dt, int n_part,
                                                        const double* part_bf, int n_bf, double dx, double dy,
                                                        double host_dt, double cfl, int mode) {
+    step_ctl_block(ctl, part_dt, n_part, part_bf, n_bf, dx, dy, host_dt, cfl, mode);
+}
+
+// Segment boundaries sized by cost.  A row whose x faces all took the
+// uniform-face path costs about 4/5 of one that solved them (measured per
+// wavefront on the magnetic sieve: 38-42 against 46-50 us per 17-18-row
+// segment), so row r of the wide strips costs sum over strips of (4 or 5),
+// and segment g (0 < g < n_segs) starts after the row whose cost interval
+// (P(r), P(r + 1)] holds the target g total / n_segs.  With at least two
+// rows per segment the targets lie more than one row's cost apart (a row
+// costs at most 5/4 of the mean row), so the boundaries increase strictly
+// and every segment keeps at least one row.  One workgroup; larger grids
+// than the LDS limits below, or fewer than two rows per segment, keep the
+// table as it is.
+constexpr int RB_MAX_ROWS = 8192;
+constexpr int RB_W_UNIFORM = 4, RB_W_SOLVED = 5;
+struct Rebalance {
+    const unsigned char* flags;   // [nx][RB_FLAG_PITCH]
+    int n_wide, nx, n_segs;
+    int* seg_i0;                  // [n_segs + 1]
+};
+
+__device__ __forceinline__ void rebalance_block(const Rebalance& rb) {
+    __shared__ int rcost[RB_MAX_ROWS];
+    __shared__ long wtot[4];
+    const int T = (int)blockDim.x, t = (int)threadIdx.x;
+    const int nx = rb.nx, ns = rb.n_wide, ng = rb.n_segs;
+    if (nx > RB_MAX_ROWS || nx < 2 * ng || ng < 2 || ns < 1 || ns > RB_FLAG_PITCH || T != 256) return;
+    // each thread's run of rows [r0, r1)
+    const int R = (nx + T - 1) / T;
+    const int r0 = t * R < nx ? t * R : nx, r1 = r0 + R < nx ? r0 + R : nx;
+    // row costs: the flag bytes (0 or 1, padding 0) of a row counted by
+    // popcount, 16 bytes per load, four rows' loads in flight at a time
+    const int words = (ns + 15) / 16;
+    long loc = 0;
+    for (int rb0 = r0; rb0 < r1; rb0 += 4) {
+        uint4 v[4][RB_FLAG_PITCH / 16];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+#pragma unroll
+            for (int w = 0; w < RB_FLAG_PITCH / 16; ++w) {
+                const bool ok = rb0 + u < r1 && w < words;
+                v[u][w] = ok ? *(const uint4*)(rb.flags + (long)(rb0 + u) * RB_FLAG_PITCH + 16 * w) : make_uint4(0, 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            int uni = 0;
+#pragma unroll
+            for (int w = 0; w < RB_FLAG_PITCH / 16; ++w)
+                uni += __popc(v[u][w].x) + __popc(v[u][w].y) + __popc(v[u][w].z) + __popc(v[u][w].w);
+            if (rb0 + u < r1) {
+                const int c = uni * RB_W_UNIFORM + (ns - uni) * RB_W_SOLVED;
+                rcost[rb0 + u] = c;
+                loc += c;
+            }
+        }
+    }
+    // exclusive prefix of the runs: inclusive scan in each wavefront, then
+    // the wavefront totals
+    const int lane = t & 63, wv = t >> 6;
+    long inc = loc;
+    for (int d = 1; d < 64; d <<= 1) {
+        const long o = __shfl_up(inc, d, 64);
+        if (lane >= d) inc += o;
+    }
+    if (lane == 63) wtot[wv] = inc;
+    __syncthreads();
+    long P = inc - loc, total = 0;
+    for (int u = 0; u < 4; ++u) {
+        if (u < wv) P += wtot[u];
+        total += wtot[u];
+    }
+    for (int r = r0; r < r1; ++r) {
+        const long Pn = P + rcost[r];
+        for (long g = P * ng / total + 1; g <= Pn * ng / total && g < ng; ++g) rb.seg_i0[g] = r + 1;
+        P = Pn;
+    }
+    if (t == 0) {
+        rb.seg_i0[0] = 0;
+        rb.seg_i0[ng] = nx;
+    }
+}
+
+// Step control with the segment boundaries of the next step re-sized from
+// this step's row flags.
+__global__ __launch_bounds__(256) void step_ctl_balance_kernel(StepCtl* ctl, const double* part_dt, int n_part,
+                                                               const double* part_bf, int n_bf, double dx, double dy,
+                                                               double host_dt, double cfl, int mode, Rebalance rb) {
+    rebalance_block(rb);
     step_ctl_block(ctl, part_dt, n_part, part_bf, n_bf, dx, dy, host_dt, cfl, mode);
 }
 

@@ -68,6 +68,14 @@ int amhd_get_rows(void* ctx);
  * on it. */
 int amhd_set_packing(void* ctx, int max_groups);
 
+/* Cost-sized segments (no counterpart in the reference): each step the
+ * last stage records which rows' x faces were all uniform (cheap: the
+ * physical flux instead of the HLLD solve), and the step control moves the
+ * segment boundaries so that every wavefront's segment costs about the
+ * same.  Default 1 for FP32 contexts (where uniform faces are cheaper), 0
+ * for FP64.  Results do not depend on it. */
+int amhd_set_balance(void* ctx, int on);
+
 /* MHDSolver.U assignment / read (solver.py:83, 139).  U: ghosted float64
  * (10, nx+4, ny+4), host or device memory.  On read, ghost cells are filled
  * as apply_bc (solver.py:158-197) leaves them. */
