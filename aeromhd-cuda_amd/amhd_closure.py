@@ -614,6 +614,16 @@ def _dvec(v):
     return (C.c_double * len(v))(*v)
 
 
+def check_closure_boundaries(cfg) -> None:
+    """The closure's kernels (features, face corrections, limiters, guard)
+    wrap y periodically and carry no correction through the x boundaries,
+    the boundaries of the RMI runs it is trained on; other boundary
+    conditions would wrap or drop correction fluxes and break conservation."""
+    if cfg.bc_y_type != "periodic" or cfg.get_bc_x() == "periodic":
+        raise ValueError("the learned closure needs periodic y and non-periodic x boundaries "
+                         f"(got bc_x {cfg.get_bc_x()!r}, bc_y {cfg.bc_y_type!r})")
+
+
 class ClosureSolver:
     """Coarse MHDSolver with the learned flux correction, its limiters and
     the plain-step fallback.  Every launch of a corrected step runs on the
@@ -629,6 +639,7 @@ class ClosureSolver:
         (None: no limiters)."""
         from solver import MHDSolver
 
+        check_closure_boundaries(cfg)
         self.s = MHDSolver(cfg)
         self.model = model
         self.fallbacks = 0
@@ -716,12 +727,17 @@ class BatchClosure:
         cfgs = [c for c, _ in cases]
         if len({(c.nx, c.ny) for c in cfgs}) != 1:
             raise ValueError("BatchClosure: every instance must have the same grid")
+        for c in cfgs:
+            check_closure_boundaries(c)
         self.b = BatchSolver(cases, precision=cfgs[0].precision)
         self.model = model
         self.n = len(cases)
         self.fallbacks = np.zeros(self.n, dtype=np.int64)
         dtype = _L.AMHD_F32 if self.b.precision in ("fp32", "float32") else _L.AMHD_F64
-        self.stream = _L.stream_handle(torch)
+        # the corrector's launches go to the stream current at construction;
+        # the end of each step waits for that stream itself
+        self.tstream = torch.cuda.current_stream()
+        self.stream = C.c_void_p(self.tstream.cuda_stream)
         self.cor = (_Corrector(model, self.n, cfgs[0], dtype, self.stream, keep, divb_floor, fields, gain)
                     if model is not None else None)
 
@@ -758,8 +774,9 @@ class BatchClosure:
             post, _ = self._ptrs()
             ok, div0 = self.cor.correct(pre, post, dt, div0)
             self.fallbacks += ((~ok) & (taken > 0)).astype(np.int64)
-            # the next step reads the corrected states on the batch's stream
-            torch.cuda.current_stream().synchronize()
+            # the next step reads the corrected states on the batch's stream:
+            # wait for the corrector's stream (whichever stream is current now)
+            self.tstream.synchronize()
 
 
 # ------------------------------------------------------- evaluation
@@ -911,6 +928,70 @@ def evaluate_batched(seeds, model, steps: int = 2000, nx: int = 2048, ny: int = 
     if log:
         log("[closure] batched evaluation: " + ", ".join(f"{k}={v:.4g}" if isinstance(v, float) else f"{k}={v}"
                                                       for k, v in out.items() if k != "seeds"))
+    return out
+
+
+def evaluate_resolutions(seeds, factors=(4, 2), steps: int = 2000, nx: int = 2048, ny: int = 1024, reps: int = 3,
+                         log=None) -> dict:
+    """The plain solver (no network) at finer grids than the closure's: for
+    each coarsening factor f (4: 512x256, 2: 1024x512), the seeds' fine
+    initial states restricted by f advance as one plain FP32 batch to their
+    fine runs' end times; the result, restricted to the closure's grid
+    (FACTOR), is scored as evaluate_batched scores the closure (integral
+    mixing width and L1 density error against the restricted fine state).
+    Wall time as in evaluate_batched: the median of `reps` timed runs after
+    an untimed 2-step run.  This gives the plain solver's cost of reaching a
+    given error, against which the closure's cost at its error compares."""
+    torch = _L.require_gpu()
+    from amhd_batch import BatchSolver
+    from amhd_diag import mixing_width
+
+    refs, fines = [], []
+    for seed in seeds:
+        fine = {}
+
+        def grab0(s):
+            fine["U0"] = s.ctx.download()
+
+        s = fine_run(seed, nx, ny, steps, grab0, [0])
+        Uf = s.ctx.download()
+        ccfg = coarse_config(s.cfg)
+        refs.append((restrict(Uf)[0].cpu().numpy(), mixing_width(ghosted(restrict(Uf)).cpu().numpy(), ccfg), ccfg))
+        fines.append((s.cfg, fine["U0"], s.t))
+    out = {"seeds": list(seeds)}
+    for f in factors:
+        cases = []
+        for cfg, U0, t_end in fines:
+            c = coarse_config(cfg, f)
+            c.t_end, c.max_steps = t_end, 10 ** 7
+            cases.append((c, ghosted(restrict(U0, f)).cpu().numpy()))
+        warm = []
+        for c, U in cases:
+            wc = Config(**{k: getattr(c, k) for k in c.__dataclass_fields__})
+            wc.max_steps = 2
+            warm.append((wc, U))
+        BatchSolver(warm).run(10)
+        walls = []
+        for _ in range(max(1, reps)):
+            b = BatchSolver(cases)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            while b.run(64).sum() > 0:
+                pass
+            torch.cuda.synchronize()
+            walls.append(time.perf_counter() - t0)
+        mw_err, l1 = [], []
+        for k, (rho_f, mw_fr, ccfg) in enumerate(refs):
+            Uc = restrict(torch.from_numpy(b.download(k)), FACTOR // f).numpy()
+            mw_err.append(abs(mixing_width(ghosted(torch.from_numpy(Uc).cuda()).cpu().numpy(), ccfg) - mw_fr)
+                          / max(abs(mw_fr), 1e-30))
+            l1.append(float(np.abs(Uc[0] - rho_f).mean() / np.abs(rho_f).mean()))
+        tag = f"{nx // f}x{ny // f}"
+        out[tag] = {"factor": f, "wall_s": float(np.median(walls)), "wall_runs_s": [round(w, 4) for w in walls],
+                    "mw_err_mean": float(np.mean(mw_err)), "l1rho_mean": float(np.mean(l1)),
+                    "steps": int(b.steps.sum())}
+        if log:
+            log(f"[closure] plain {tag}: " + ", ".join(f"{k}={v}" for k, v in out[tag].items()))
     return out
 
 

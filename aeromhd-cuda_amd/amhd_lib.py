@@ -65,7 +65,7 @@ SIGNATURES = {
     "amhd_cons_to_prim": (_i, [_vp, _vp, _l, _d, _vp]),
     "amhd_prim_to_cons": (_i, [_vp, _vp, _l, _d, _vp]),
     "amhd_muscl": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp]),
-    "amhd_hlld_flux": (_i, [_vp, _vp, _vp, _dp, _l, _i, _d, _d, _d, _d, _i, _vp]),
+    "amhd_hlld_flux": (_i, [_vp, _vp, _vp, _vp, _dp, _l, _i, _d, _d, _d, _d, _i, _vp]),
     "amhd_apply_bc": (_i, [_vp, _i, _i, _i, _i, _vp]),
     "amhd_enforce_scalar_bounds": (_i, [_vp, _l, _vp]),
     "amhd_compute_dt_w": (_i, [_vp, _l, _d, _d, _d, _d, _dp, _dp, _vp]),

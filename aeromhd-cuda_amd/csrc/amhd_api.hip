@@ -940,8 +940,8 @@ int amhd_muscl(const double* W, int nxt, int nyt, int dir, double* WL, double* W
     return 0;
 }
 
-int amhd_hlld_flux(const void* WL, const void* WR, void* F, double* smax, long n, int dir, double gamma, double ch,
-                   double floor_rho, double floor_pr, int dtype, void* stream) {
+int amhd_hlld_flux(const void* WL, const void* WR, void* F, void* smax_arr, double* smax, long n, int dir,
+                   double gamma, double ch, double floor_rho, double floor_pr, int dtype, void* stream) {
     if (!WL || !WR || !F || n < 0) return fail(-1, "amhd_hlld_flux: bad argument");
     if (dir != 0 && dir != 1) return fail(-1, "amhd_hlld_flux: dir must be 0 or 1");
     hipStream_t s = (hipStream_t)stream;
@@ -957,10 +957,10 @@ int amhd_hlld_flux(const void* WL, const void* WR, void* F, double* smax, long n
         if (dtype == AMHD_F64) {
             const FluxConst<double> kc = make_flux_const<double>(gamma, ch, floor_rho, floor_pr);
             hlld_kernel<double, ExactMath<double>><<<grid_for(n, 256), 256, 0, s>>>(
-                (const double*)WL, (const double*)WR, (double*)F, n, dir, kc, bits);
+                (const double*)WL, (const double*)WR, (double*)F, (double*)smax_arr, n, dir, kc, bits);
             HIP_TRY(hipGetLastError());
         } else if (dtype == AMHD_F32) {
-            HIP_TRY(launch_hlld_f32((const float*)WL, (const float*)WR, (float*)F, n, dir, gamma, ch, floor_rho,
+            HIP_TRY(launch_hlld_f32((const float*)WL, (const float*)WR, (float*)F, (float*)smax_arr, n, dir, gamma, ch, floor_rho,
                                     floor_pr, bits, s));
         } else {
             return fail(-1, "amhd_hlld_flux: unknown dtype %d", dtype);

@@ -40,10 +40,10 @@ hipError_t launch_materialize_f32(const float* u, long plane, int pitch, int nx,
     return launch_materialize_impl<float>(u, plane, pitch, nx, ny, bcx, bcy, ug, s);
 }
 
-hipError_t launch_hlld_f32(const float* wl, const float* wr, float* f, long n, int dir, double gamma, double ch,
-                           double floor_rho, double floor_pr, unsigned long long* smax_bits, hipStream_t s) {
+hipError_t launch_hlld_f32(const float* wl, const float* wr, float* f, float* smax_arr, long n, int dir, double gamma,
+                           double ch, double floor_rho, double floor_pr, unsigned long long* smax_bits, hipStream_t s) {
     const FluxConst<float> kc = make_flux_const<float>(gamma, ch, floor_rho, floor_pr);
-    hlld_kernel<float, FastMath<float>><<<grid_for(n, 256), 256, 0, s>>>(wl, wr, f, n, dir, kc, smax_bits);
+    hlld_kernel<float, FastMath<float>><<<grid_for(n, 256), 256, 0, s>>>(wl, wr, f, smax_arr, n, dir, kc, smax_bits);
     return hipGetLastError();
 }
 

@@ -1013,18 +1013,21 @@ __global__ void muscl_kernel(const double* __restrict__ W, int nxt, int nyt, int
 
 #endif  // AMHD_API_TU
 
-// physics.py:43-275 + 353-379 over (NV, n) face arrays; smax as the bit
-// pattern of a non-negative double (ordered like the value) for atomicMax.
+// physics.py:43-275 + 353-379 over (NV, n) face arrays; smax_arr (may be
+// null) receives every face's max(|SL|, |SR|) (physics.py:108-109), smax
+// the maximum as the bit pattern of a non-negative double (ordered like the
+// value) for atomicMax.
 template <class real, class P>
 __global__ __launch_bounds__(256) void hlld_kernel(const real* __restrict__ WL, const real* __restrict__ WR,
-                                                   real* __restrict__ F, long n, int dir, FluxConst<real> kc,
-                                                   unsigned long long* smax_bits) {
+                                                   real* __restrict__ F, real* __restrict__ smax_arr, long n, int dir,
+                                                   FluxConst<real> kc, unsigned long long* smax_bits) {
     double smax = 0.0;
     for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x) {
         real wl[NV], wr[NV], f[NV];
 #pragma unroll
         for (int v = 0; v < NV; ++v) { wl[v] = WL[v * n + k]; wr[v] = WR[v * n + k]; }
         const real s = dir == 0 ? hlld_face<real, P>(wl, wr, f, kc) : hlld_face_y<real, P>(wl, wr, f, kc);
+        if (smax_arr) smax_arr[k] = s;
         smax = fmax(smax, double(s));
 #pragma unroll
         for (int v = 0; v < NV; ++v) F[v * n + k] = f[v];

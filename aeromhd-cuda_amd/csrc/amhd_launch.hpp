@@ -44,8 +44,8 @@ hipError_t launch_materialize_f64(const double* u, long plane, int pitch, int nx
 hipError_t launch_materialize_f32(const float* u, long plane, int pitch, int nx, int ny, int bcx, int bcy,
                                   double* ug, hipStream_t s);
 
-hipError_t launch_hlld_f32(const float* wl, const float* wr, float* f, long n, int dir, double gamma, double ch,
-                           double floor_rho, double floor_pr, unsigned long long* smax_bits, hipStream_t s);
+hipError_t launch_hlld_f32(const float* wl, const float* wr, float* f, float* smax_arr, long n, int dir, double gamma,
+                           double ch, double floor_rho, double floor_pr, unsigned long long* smax_bits, hipStream_t s);
 
 inline int grid_for(long n, int block, int cap = 4096) {
     long g = (n + block - 1) / block;

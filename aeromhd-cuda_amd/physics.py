@@ -193,30 +193,43 @@ def muscl_y(W):
     return _muscl(W, 1)
 
 
-def _hlld(WL, WR, gamma, ch, direction: int):
+def _hlld(WL, WR, gamma, ch, direction: int, smax_arr=None):
     WLd, host = _to_device(WL)
     WRd, _ = _to_device(WR)
     if WLd.shape != WRd.shape or WLd.shape[0] != NVAR:
         raise ValueError("hlld: WL and WR must have the same shape (NVAR, ...)")
     n = WLd.numel() // NVAR
     F = _torch().empty_like(WLd)
+    sa = None
+    if smax_arr is not None:
+        if int(np.prod(np.shape(smax_arr))) != n:
+            raise ValueError(f"hlld: smax_arr must hold one value per face ({n})")
+        sa = _torch().empty(n, dtype=WLd.dtype, device=WLd.device)
     smax = C.c_double(0.0)
-    _L.call("amhd_hlld_flux", _ptr(WLd), _ptr(WRd), _ptr(F), C.byref(smax), n, direction,
-            float(gamma), float(ch), FLOOR_RHO, FLOOR_PR, _L.AMHD_F64, _stream())
+    _L.call("amhd_hlld_flux", _ptr(WLd), _ptr(WRd), _ptr(F), _ptr(sa) if sa is not None else None, C.byref(smax), n,
+            direction, float(gamma), float(ch), FLOOR_RHO, FLOOR_PR, _L.AMHD_F64, _stream())
+    if smax_arr is not None:
+        if isinstance(smax_arr, np.ndarray):
+            smax_arr[...] = sa.cpu().numpy().reshape(smax_arr.shape)
+        else:
+            smax_arr.copy_(sa.reshape(smax_arr.shape))
     return _from_device(F, host), float(smax.value)
 
 
-def hlld_flux_x(WL, WR, gamma, ch):
+def hlld_flux_x(WL, WR, gamma, ch, smax_arr=None):
     """HLLD flux through x faces, one thread per face (physics.py:353-370).
 
-    Returns ``(F, smax)`` with ``smax = max(|SL|, |SR|)`` over the faces."""
-    return _hlld(WL, WR, gamma, ch, 0)
+    Returns ``(F, smax)`` with ``smax = max(|SL|, |SR|)`` over the faces;
+    ``smax_arr`` (optional, one value per face) receives each face's
+    ``max(|SL|, |SR|)`` as the reference kernel's ``smax_arr``
+    (physics.py:108-109)."""
+    return _hlld(WL, WR, gamma, ch, 0, smax_arr)
 
 
-def hlld_flux_y(WL, WR, gamma, ch):
+def hlld_flux_y(WL, WR, gamma, ch, smax_arr=None):
     """HLLD flux through y faces by the (vx,vy),(Bx,By) rotation
     (physics.py:373-379), done inside the kernel."""
-    return _hlld(WL, WR, gamma, ch, 1)
+    return _hlld(WL, WR, gamma, ch, 1, smax_arr)
 
 
 # ---------------------------------------------------- Rankine-Hugoniot (host)

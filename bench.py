@@ -56,7 +56,8 @@ def make_workload(name: str, seed: int):
     return cfg, W, desc
 
 
-def gpu_measure(name: str, steps: int, warmup: int, seed: int, rows: int | None, packing: int | None = None):
+def gpu_measure(name: str, steps: int, warmup: int, seed: int, rows: int | None, packing: int | None = None,
+                tune: bool = False):
     import torch
     from physics import prim_to_cons
     from solver import MHDSolver
@@ -68,6 +69,10 @@ def gpu_measure(name: str, steps: int, warmup: int, seed: int, rows: int | None,
     if packing is not None:
         s.ctx.set_packing(packing)
     s.U = prim_to_cons(W, cfg.gamma)
+    if tune:
+        import amhd_tune
+
+        amhd_tune.tune_single(s, use_cache=False)   # times the candidates on a copy of this state
     s.advance(warmup)
     # timed region 1 (value): the product loop -- device dt, hipGraph replay
     torch.cuda.synchronize()
@@ -180,26 +185,50 @@ def slab_measure(n_slabs: int = 4, steps: int = 40, warmup: int = 3, seed: int =
 def closure_measure(seeds=tuple(range(200, 208)), fine_steps: int = 2000):
     """The learned closure in its batched form: 8 held-out magnetic-sieve
     seeds as one BatchClosure (coarse 256x128 FP32 instances, one batched
-    step, network pass and limited correction per step) and as the plain
-    coarse batch, each compared with its fine 2048x1024 run restricted to
-    the coarse grid; the value is the closure's fine-equivalent
-    cell-update rate over the plain FP64 fine solve's."""
+    step, network pass and limited correction per step), each compared with
+    its fine 2048x1024 run restricted to the coarse grid.
+
+    Error-matched comparison: the plain solver (no network) runs the same
+    seeds as batches at 256x128, 512x256 and 1024x512, scored on the
+    256x128 grid the same way.  The speed-up at equal error is the wall
+    time of the cheapest plain resolution whose error is at most the
+    closure's, over the closure's wall time (a lower bound when no plain
+    resolution reaches it); it is given for the integral mixing width (the
+    closure's target) and for the L1 density error.  The fine-equivalent
+    rate over the FP64 fine solve is kept as a secondary figure."""
     import amhd_closure as cl
 
     model = cl.FluxCNN.load(os.path.join(PKG, "closure_weights.npz"))
     bt = cl.evaluate_batched(list(seeds), model, steps=fine_steps)
+    res = cl.evaluate_resolutions(list(seeds), factors=(4, 2), steps=fine_steps)
     fp64 = cl.fp64_fine_throughput(seeds[0], fine_steps)
     fine_cells = 2048 * 1024 * fine_steps * len(seeds)
+    plain = [("256x128", bt["wall_plain_s"], bt["mw_err_plain_mean"], bt["l1rho_plain_mean"])]
+    for tag in ("512x256", "1024x512"):
+        plain.append((tag, res[tag]["wall_s"], res[tag]["mw_err_mean"], res[tag]["l1rho_mean"]))
+    wc, mwc, l1c = bt["wall_closure_s"], bt["mw_err_closure_mean"], bt["l1rho_closure_mean"]
+
+    def equal_error(idx, err_c):
+        ok = [p for p in plain if p[idx] <= err_c * (1 + 1e-9)]
+        if ok:
+            p = min(ok, key=lambda q: q[1])
+            return {"plain_grid": p[0], "speedup": round(p[1] / wc, 3), "bound": "exact match or better"}
+        p = plain[-1]
+        return {"plain_grid": p[0], "speedup_at_least": round(p[1] / wc, 3),
+                "bound": "no plain grid tested reaches the closure's error; the finest one's time is a lower bound"}
+
     return {"workload": f"learned closure, batched: {len(seeds)} held-out magnetic-sieve seeds "
                         f"({seeds[0]}-{seeds[-1]}) in one BatchClosure, coarse 256x128 FP32 + BF16-MFMA CNN, "
-                        "vs the FP64 fine 2048x1024 solve",
-            "value": round(fine_cells / bt["wall_closure_s"] / 1e6 / fp64["mcups"], 2),
-            "unit": "x fine-equivalent Mcell-updates/s over FP64 fine",
+                        "against plain batches at 256x128, 512x256 and 1024x512 and the FP64 fine 2048x1024 solve",
+            "equal_mixing_width_error": equal_error(2, mwc),
+            "equal_l1rho_error": equal_error(3, l1c),
+            "mw_err_closure": round(mwc, 4), "l1rho_closure": round(l1c, 5),
+            "mw_target_5pct_met": bool(mwc <= 0.05),
+            "plain": {t: {"wall_s": round(w, 4), "mw_err": round(m, 4), "l1rho": round(l, 5)} for t, w, m, l in plain},
+            "fine_equiv_over_fp64_fine": round(fine_cells / wc / 1e6 / fp64["mcups"], 2),
             "fp64_fine_mcups": round(fp64["mcups"], 1),
-            "mw_err_closure": round(bt["mw_err_closure_mean"], 4), "mw_err_plain": round(bt["mw_err_plain_mean"], 4),
-            "l1rho_closure": round(bt["l1rho_closure_mean"], 5), "l1rho_plain": round(bt["l1rho_plain_mean"], 5),
-            "fallbacks": bt["fallbacks_closure"], "wall_closure_s": round(bt["wall_closure_s"], 4),
-            "wall_plain_s": round(bt["wall_plain_s"], 4), "wall_closure_runs_s": bt["wall_closure_runs_s"],
+            "fallbacks": bt["fallbacks_closure"], "wall_closure_s": round(wc, 4),
+            "wall_closure_runs_s": bt["wall_closure_runs_s"],
             "timing": "median of 3 timed runs after an untimed 2-step run; FP64 fine timed after 5 warm-up steps"}
 
 
@@ -322,14 +351,26 @@ def main():
     log(f"[bench] {m['mcups']:.1f} Mcell/s, stage us {m['stage_avg_us']}, achieved {m['achieved_gbs']:.0f} GB/s")
 
     secondary = []
+    if not args.no_secondary and args.workload == "sieve2048":
+        # the same workload late in the run: the mixing layer has developed
+        # and fewer rows are uniform in x
+        log("[bench] secondary sieve2048 late (after 1500 untimed steps)")
+        x = gpu_measure("sieve2048", args.steps, 1500, args.seed, args.rows or None, args.packing)
+        secondary.append({"workload": x["desc"] + ", timed after 1500 untimed steps", "value": round(x["mcups"], 2),
+                          "unit": "Mcell-updates/s", "dtype": "f32", "ms_per_step": round(x["ms_per_step"], 4),
+                          "achieved_GBps": round(x["achieved_gbs"], 1),
+                          "frac": round(x["achieved_gbs"] / HBM_PEAK_GBS, 4),
+                          "stage_avg_us": [round(u, 2) for u in x["stage_avg_us"]], "t": x["t"]})
     if not args.no_secondary:
         for wl in ("rmi1024-fp64", "rmi1024-fp32"):
             if wl == args.workload:
                 continue
             log(f"[bench] secondary {wl}")
-            x = gpu_measure(wl, 200, 20, args.seed, None)
+            d = gpu_measure(wl, 200, 20, args.seed, None)
+            x = gpu_measure(wl, 200, 20, args.seed, None, tune=True)
             secondary.append({"workload": x["desc"], "value": round(x["mcups"], 2), "unit": "Mcell-updates/s",
                               "dtype": "f32" if x["esize"] == 4 else "f64",
+                              "rows_per_segment_tuned": x["rows"], "value_default_rows": round(d["mcups"], 2),
                               "ms_per_step": round(x["ms_per_step"], 4),
                               "achieved_GBps": round(x["achieved_gbs"], 1),
                               "frac": round(x["achieved_gbs"] / HBM_PEAK_GBS, 4),
@@ -354,7 +395,8 @@ def main():
         cpu = cpu_baseline(args.seed, args.cpu_steps, threads, args.cpu_steps_1core)
 
     copy_gbs = copy_bandwidth()
-    log(f"[bench] device copy {copy_gbs:.0f} GB/s")
+    copy_ws_gbs = copy_bandwidth(mib=80, reps=50)
+    log(f"[bench] device copy {copy_gbs:.0f} GB/s (1 GiB), {copy_ws_gbs:.0f} GB/s (80 MiB, one state's size)")
     traffic = traffic_from_profile(args.workload)
     avg_launch_bytes = sum(m["bytes_per_launch"]) / 3
     out = {
@@ -381,7 +423,8 @@ def main():
                      "stage_avg_us": [round(u, 2) for u in m["stage_avg_us"]],
                      "ctl_avg_us": round(m["ctl_avg_us"], 2),
                      "copy_GBps": round(copy_gbs, 1),
-                     "frac_of_copy": round(m["achieved_gbs"] / copy_gbs, 4)},
+                     "frac_of_copy": round(m["achieved_gbs"] / copy_gbs, 4),
+                     "copy_GBps_80MiB": round(copy_ws_gbs, 1)},
         "cpu_baseline": cpu,
         "profiled_pass_value": round(m["profiled_mcups"], 2),
         "secondary": secondary,

@@ -180,11 +180,13 @@ int amhd_muscl(const double* W, int nxt, int nyt, int dir, double* WL, double* W
 /* The HLLD kernel hlld_flux_x_kernel(WL, WR, F, smax_arr, n_faces, gamma,
  * ch, floor_rho, floor_pr) of physics.py:44-55 with the wrappers of
  * physics.py:353-379: (10, n_faces) planes, dir 0 = x faces, dir 1 = y faces
- * (vx<->vy, Bx<->By rotation).  smax (may be NULL) receives
- * max(|SL|,|SR|) over the faces, as float(xp.max(smax_arr)) does
- * (synchronises).  dtype AMHD_F64 or AMHD_F32 for WL/WR/F. */
-int amhd_hlld_flux(const void* WL, const void* WR, void* F, double* smax, long n_faces, int dir, double gamma,
-                   double ch, double floor_rho, double floor_pr, int dtype, void* stream);
+ * (vx<->vy, Bx<->By rotation).  smax_arr (device, n_faces values, may be
+ * NULL) receives each face's max(|SL|,|SR|) as the reference kernel writes
+ * it (physics.py:108-109); smax (host, may be NULL) their maximum, as
+ * float(xp.max(smax_arr)) does (synchronises).  dtype AMHD_F64 or AMHD_F32
+ * for WL/WR/F/smax_arr. */
+int amhd_hlld_flux(const void* WL, const void* WR, void* F, void* smax_arr, double* smax, long n_faces, int dir,
+                   double gamma, double ch, double floor_rho, double floor_pr, int dtype, void* stream);
 
 /* solver.py:158-197 in place on a ghosted (10, nx+4, ny+4) device array. */
 int amhd_apply_bc(double* U, int nx, int ny, int bc_x_periodic, int bc_y_periodic, void* stream);

@@ -84,6 +84,29 @@ def test_hlld_matches_reference_kernel_on_gpu(torch_gpu, orc):
         assert smax == sr.max()
 
 
+@pytest.mark.parametrize("case", ["brio_wu_faces", "degenerate_ch0", "degenerate_ch3", "random_ch0",
+                                  "random_ch0p5_g2", "random_ch5"])
+def test_hlld_per_face_smax_matches_reference_vectors(torch_gpu, case):
+    """The per-face smax_arr of the reference kernel (physics.py:108-109), as
+    stored with the golden vectors from its compiled source: the product's
+    per-face output equals it bit for bit, x faces and (with the vx/vy,
+    Bx/By rotation) y faces, and the returned maximum is its maximum."""
+    import physics
+
+    g = np.load(os.path.join(REPO, "tests", "golden", "hlld_ref_vectors.npz"))
+    WL, WR, smax_ref = g[case + "__WL"], g[case + "__WR"], g[case + "__smax"]
+    gamma, ch = float(g[case + "__gamma"]), float(g[case + "__ch"])
+    sa = np.full(WL.shape[1], np.nan)
+    F, smax = physics.hlld_flux_x(WL, WR, gamma, ch, smax_arr=sa)
+    np.testing.assert_array_equal(sa, smax_ref)
+    np.testing.assert_array_equal(F, g[case + "__F"])
+    assert smax == smax_ref.max()
+    rot = [0, 2, 1, 3, 5, 4, 6, 7, 8, 9]
+    sy = np.full(WL.shape[1], np.nan)
+    physics.hlld_flux_y(WL[rot], WR[rot], gamma, ch, smax_arr=sy)
+    np.testing.assert_array_equal(sy, smax_ref)
+
+
 def test_conversions_muscl_bc_bounds_dt_bitwise(torch_gpu, orc):
     import physics
     from amhd_ics import brio_wu

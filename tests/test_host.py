@@ -166,3 +166,17 @@ def test_batched_draws_reproducible_and_in_range():
         np.testing.assert_array_equal(wa, wb)
         assert ca.nx % 64 == 0 and ca.ny == ca.nx // 2 and 2 <= ca.mach <= 12
         assert 0.2 <= (ca.density_ratio - 1) / (ca.density_ratio + 1) <= 0.8
+
+
+def test_closure_rejects_unsupported_boundaries():
+    """The closure's kernels assume periodic y and non-periodic x boundaries
+    (the RMI runs it is trained on); other configurations are refused before
+    any device work."""
+    from amhd_closure import check_closure_boundaries
+    from config import Config
+
+    check_closure_boundaries(Config(bc_x_type="extrapolation", bc_y_type="periodic"))
+    check_closure_boundaries(Config(bc_x_type="auto", use_char_bc=True, bc_y_type="periodic"))
+    for bx, by in (("periodic", "periodic"), ("extrapolation", "outflow"), ("characteristic", "extrapolation")):
+        with pytest.raises(ValueError, match="periodic y"):
+            check_closure_boundaries(Config(bc_x_type=bx, bc_y_type=by))
