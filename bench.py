@@ -16,7 +16,7 @@ instances), the x-slab decomposition (4 slabs with halo exchange, one
 GPU), and the learned closure's fine-equivalent rate over the FP64 fine
 solve with its mixing-width and density errors.
 
-  python bench.py [--steps K] [--warmup W] [--workload sieve2048|rmi1024-fp64|rmi1024-fp32]
+  python bench.py [--steps K] [--warmup W] [--workload sieve2048|sieve2048-fp64|rmi1024-fp64|rmi1024-fp32]
 """
 from __future__ import annotations
 
@@ -42,9 +42,11 @@ def log(*a):
 def make_workload(name: str, seed: int):
     from amhd_ics import magnetic_sieve, rmi_uniform_bx
 
-    if name == "sieve2048":
-        cfg, W = magnetic_sieve(2048, 1024, precision="fp32", seed=seed)
-        desc = f"magnetic-sieve RMI 2048x1024 FP32 (seed {seed}): MUSCL+HLLD+GLM+Powell, SSP-RK3, device dt"
+    if name in ("sieve2048", "sieve2048-fp64"):
+        prec = "fp64" if name.endswith("fp64") else "fp32"
+        cfg, W = magnetic_sieve(2048, 1024, precision=prec, seed=seed)
+        desc = (f"magnetic-sieve RMI 2048x1024 {prec.upper()} (seed {seed}): MUSCL+HLLD+GLM+Powell, SSP-RK3, "
+                "device dt")
     elif name in ("rmi1024-fp64", "rmi1024-fp32"):
         prec = name.split("-")[1]
         cfg, W = rmi_uniform_bx(1024, 512, precision=prec)
