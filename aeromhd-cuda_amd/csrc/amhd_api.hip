@@ -106,15 +106,19 @@ int noalloc_scratch(double** out) {
     return 0;
 }
 
-// Most segments of a narrow last strip per wavefront (the default where
-// two or more stage wavefronts share a SIMD).  Each lane group of a packed
-// wavefront is on its own row, so its loads and stores touch a cache line
-// per group: measured on the 2048 x 1024 FP32 sieve (4 valid cells in the
-// last strip), packed wavefronts ran about 5% longer than the others with 2
-// groups, 8% with 4 and 25% with 8 (stages 2 and 3, which also read U0),
-// which only 2 groups repay.
+// Most segments of a narrow last strip per wavefront (default 0: off; a
+// tuning switch).  Each lane group of a packed wavefront is on its own row,
+// so its loads and stores touch a cache line per group: measured on the
+// 2048 x 1024 FP32 sieve (4 valid cells in the last strip), packed
+// wavefronts ran about 5% longer than the others with 2 groups, 8% with 4
+// and 25% with 8 (stages 2 and 3, which also read U0).  With equal-row
+// segments 2 groups still gained 1%; with the cost-sized segments the packed
+// wavefronts (whose x faces take the uniform-face path only where both
+// groups' rows are uniform, and which carry no row flags) ended last in
+// every launch, and the sieve ran 3% faster without packing.  FP64 (one
+// wavefront per SIMD) lost 10% with 2 groups.
 #ifndef AMHD_PACK_MAX_GROUPS
-#define AMHD_PACK_MAX_GROUPS 2
+#define AMHD_PACK_MAX_GROUPS 0
 #endif
 
 struct Ctx {
@@ -284,9 +288,9 @@ hipError_t enqueue_ctl(Ctx* c, int n_part, double host_dt, double cfl, int mode,
     const double* bf = c->p.bc_x_periodic ? nullptr : c->part_bf;
     if (c->balance && mode != CTL_DT_ONLY) {
         // the step's last stage has recorded its row flags: size the next
-        // step's segments
+        // step's segments (second workgroup, beside the step control)
         Rebalance rb{c->rowflags, c->pk_groups ? c->n_strips - 1 : c->n_strips, c->nx, c->n_segs, c->seg_i0};
-        step_ctl_balance_kernel<<<1, 256, 0, c->stream>>>(ctl, c->part_dt, n_part, bf, c->n_strips, c->p.dx, c->p.dy,
+        step_ctl_balance_kernel<<<2, 256, 0, c->stream>>>(ctl, c->part_dt, n_part, bf, c->n_strips, c->p.dx, c->p.dy,
                                                           host_dt, cfl, mode, rb);
     } else {
         step_ctl_kernel<<<1, 256, 0, c->stream>>>(ctl, c->part_dt, n_part, bf, c->n_strips, c->p.dx, c->p.dy, host_dt,
@@ -429,10 +433,6 @@ int amhd_create(void** out, int nx, int ny, int dtype, const amhd_params* p) {
                                              : stage_waves_per_cu_f32(p->powell != 0);
             c->slots = cus * wpc;
             c->cus = cus;
-            // packing pays only where a second wavefront on the SIMD hides
-            // the packed wavefront's longer memory instructions (measured on
-            // the 2048 x 1024 sieve: FP32, 2 per SIMD, +1%; FP64, 1 per SIMD, -10%)
-            if (wpc < 8) c->pack_max = 0;
         }
     }
     set_tiling(c, 0);
@@ -508,6 +508,21 @@ int amhd_set_balance(void* ctx, int on) {
     drop_graphs(c);
     return 0;
 }
+
+#ifdef AMHD_WAVE_TIMES
+// Diagnostic builds only: the context's segment table (n_segs + 1 ints) and
+// row flags (nx * RB_FLAG_PITCH bytes) copied to host memory.  Not part of
+// the ABI of include/aeromhd.h.
+int amhd_debug_tables(void* ctx, int* seg_i0, unsigned char* flags, int* n_segs) {
+    if (!ctx || !seg_i0 || !flags || !n_segs) return fail(-1, "amhd_debug_tables: null argument");
+    Ctx* c = (Ctx*)ctx;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    *n_segs = c->n_segs;
+    HIP_TRY(hipMemcpy(seg_i0, c->seg_i0, sizeof(int) * (size_t)(c->n_segs + 1), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(flags, c->rowflags, (size_t)RB_FLAG_PITCH * c->nx, hipMemcpyDeviceToHost));
+    return 0;
+}
+#endif
 
 int amhd_set_packing(void* ctx, int max_groups) {
     if (!ctx) return fail(-1, "amhd_set_packing: null context");

@@ -819,18 +819,23 @@ __global__ __launch_bounds__(256) void step_ctl_kernel(StepCtl* ctl, const doubl
 }
 
 // Segment boundaries sized by cost.  A row whose x faces all took the
-// uniform-face path costs about 4/5 of one that solved them (measured per
-// wavefront on the magnetic sieve: 38-42 against 46-50 us per 17-18-row
-// segment), so row r of the wide strips costs sum over strips of (4 or 5),
-// and segment g (0 < g < n_segs) starts after the row whose cost interval
-// (P(r), P(r + 1)] holds the target g total / n_segs.  With at least two
-// rows per segment the targets lie more than one row's cost apart (a row
-// costs at most 5/4 of the mean row), so the boundaries increase strictly
-// and every segment keeps at least one row.  One workgroup; larger grids
-// than the LDS limits below, or fewer than two rows per segment, keep the
-// table as it is.
+// uniform-face path costs about 5/7 of one that solved them (a least-squares
+// fit of per-wavefront durations on the magnetic sieve, 18000 wavefronts at
+// steps 200, 800 and 1500: 1.9-2.0 us per uniform row against 2.66-2.74 us
+// per solved row, plus 5-8 us per wavefront), so row r of the wide strips
+// costs sum over strips of (5 or 7), and segment g (0 < g < n_segs) starts
+// after the row whose cost interval (P(r), P(r + 1)] holds the target
+// g total / n_segs.  With at least two rows per segment the targets lie
+// more than one row's cost apart (a row costs at most 7/5 of the mean row,
+// and 2 x 5 > 7), so the boundaries increase strictly and every segment
+// keeps at least one row.  One workgroup; larger grids than the LDS limits
+// below, or fewer than two rows per segment, keep the table as it is.
 constexpr int RB_MAX_ROWS = 8192;
-constexpr int RB_W_UNIFORM = 4, RB_W_SOLVED = 5;
+#ifndef AMHD_RB_W_UNIFORM
+#define AMHD_RB_W_UNIFORM 5
+#define AMHD_RB_W_SOLVED 7
+#endif
+constexpr int RB_W_UNIFORM = AMHD_RB_W_UNIFORM, RB_W_SOLVED = AMHD_RB_W_SOLVED;
 struct Rebalance {
     const unsigned char* flags;   // [nx][RB_FLAG_PITCH]
     int n_wide, nx, n_segs;
@@ -899,12 +904,16 @@ __device__ __forceinline__ void rebalance_block(const Rebalance& rb) {
     }
 }
 
-// Step control with the segment boundaries of the next step re-sized from
-// this step's row flags.
+// Step control (workgroup 0) and, beside it, the segment boundaries of the
+// next step re-sized from this step's row flags (workgroup 1).  The two
+// touch disjoint memory, so they run at once on two CUs.
 __global__ __launch_bounds__(256) void step_ctl_balance_kernel(StepCtl* ctl, const double* part_dt, int n_part,
                                                                const double* part_bf, int n_bf, double dx, double dy,
                                                                double host_dt, double cfl, int mode, Rebalance rb) {
-    rebalance_block(rb);
+    if (blockIdx.x == 1) {
+        rebalance_block(rb);
+        return;
+    }
     step_ctl_block(ctl, part_dt, n_part, part_bf, n_bf, dx, dy, host_dt, cfl, mode);
 }
 

@@ -89,8 +89,8 @@ class Context:
 
     def set_packing(self, max_groups: int):
         """Let up to max_groups (2, 4 or 8; 0 = off) segments of a narrow
-        last y strip share one wavefront (default: 2 for FP32 contexts, 0 for
-        FP64); the results do not depend on it."""
+        last y strip share one wavefront (default 0); the results do not
+        depend on it."""
         _L.check(self.lib.amhd_set_packing(self.h, int(max_groups)), "amhd_set_packing")
 
     def set_balance(self, on: bool):

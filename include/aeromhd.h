@@ -62,8 +62,7 @@ int amhd_get_rows(void* ctx);
 
 /* Packing of a narrow last y strip: when the last strip of 60 columns
  * holds at most 28 cells, up to max_groups of its x segments (2, 4 or 8;
- * 0 = no packing) share one wavefront.  Default: 2 where two stage
- * wavefronts share a SIMD (FP32), else 0 (FP64).  A tuning switch like
+ * 0 = no packing) share one wavefront.  Default: 0.  A tuning switch like
  * amhd_set_rows (no counterpart in the reference); results do not depend
  * on it. */
 int amhd_set_packing(void* ctx, int max_groups);
