@@ -337,23 +337,36 @@ __device__ __forceinline__ void stage_wave(const StageParams<real>& a, const int
         cons_to_prim<real, P>(QB, B, kc.gamma, psi_ghost(i0 + 1));
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
-            const real dmm = wm1[v] - wm2[v];
             dm0[v] = A[v] - wm1[v];
             dxp[v] = B[v] - A[v];
-            const real h_m1 = vl_half_d<real, P>(dmm, dm0[v]);
-            const real h_0 = vl_half_d<real, P>(dm0[v], dxp[v]);
-            fa[v] = wm1[v] + h_m1;
-            fb[v] = A[v] - h_0;
-            xl[v] = A[v] + h_0;
         }
-        floor_face<real, P>(fa); floor_face<real, P>(fb); floor_face<real, P>(xl);
-        hlld_face<real, P, GLM, true>(fa, fb, F0, kc);
-        if constexpr (XSHORT) {
-            // the uniform-face rule of the row loop, for face i0-1/2
-            if (all_zero(dm0)) {
-                phys_flux_x(A, F0, kc);
+        // the uniform-face rule of the row loop, for face i0-1/2: a lane whose
+        // face is uniform gets F(W_i0) and the left state W_i0 of the next
+        // face, whichever branch its wavefront takes
+        bool uni0 = false;
+        if constexpr (XSHORT) uni0 = all_zero(dm0);
+        if (XSHORT && __builtin_amdgcn_ballot_w64(!uni0) == 0) {
+            phys_flux_x(A, F0, kc);
 #pragma unroll
-                for (int v = 0; v < NV; ++v) xl[v] = A[v];
+            for (int v = 0; v < NV; ++v) xl[v] = A[v];
+        } else {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                const real dmm = wm1[v] - wm2[v];
+                const real h_m1 = vl_half_d<real, P>(dmm, dm0[v]);
+                const real h_0 = vl_half_d<real, P>(dm0[v], dxp[v]);
+                fa[v] = wm1[v] + h_m1;
+                fb[v] = A[v] - h_0;
+                xl[v] = A[v] + h_0;
+            }
+            floor_face<real, P>(fa); floor_face<real, P>(fb); floor_face<real, P>(xl);
+            hlld_face<real, P, GLM, true>(fa, fb, F0, kc);
+            if constexpr (XSHORT) {
+                if (uni0) {
+                    phys_flux_x(A, F0, kc);
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) xl[v] = A[v];
+                }
             }
         }
         bxm = wm1[iBX];
