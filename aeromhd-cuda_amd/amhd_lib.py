@@ -53,6 +53,7 @@ SIGNATURES = {
     "amhd_compute_rhs": (_i, [_vp, _vp, _vp, _d]),
     "amhd_global_sums": (_i, [_vp, _dp]),
     "amhd_concentration_profile": (_i, [_vp, _dp, _dp]),
+    "amhd_flow_diagnostics": (_i, [_vp, _dp, _dp, _dp]),
     "amhd_set_halo": (_i, [_vp, _vp, _vp]),
     "amhd_stage": (_i, [_vp, _i, _d, _d, _d]),
     "amhd_edge_rows": (_i, [_vp, _i, _i, _vp]),

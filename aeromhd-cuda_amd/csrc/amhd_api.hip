@@ -402,6 +402,137 @@ __global__ __launch_bounds__(256) void conc_profile_kernel(const real* __restric
     }
 }
 
+// Primitive values the flow diagnostics read (solver.py:338-347 through
+// cons_to_prim, physics.py:284-301, with its 1e-12 floors), in FP64.
+template <class real>
+struct DiagCell {
+    const real* u;
+    long plane;
+    int pitch;
+    __device__ double rho(long o) const { return fmax(double(u[RHO * plane + o]), 1e-12); }
+    __device__ double vel(int comp, long o) const { return double(u[comp * plane + o]) * (1.0 / rho(o)); }
+    __device__ double conc(long o) const {
+        const double c = double(u[RHOC * plane + o]) * (1.0 / rho(o));
+        return c < 0.0 ? 0.0 : (c > 1.0 ? 1.0 : c);   // np.clip, NaN passes
+    }
+};
+
+// Per interior row i: out[i] = sum over y of rho omega^2 with omega =
+// dvy/dx - dvx/dy as np.gradient forms it (central differences inside,
+// one-sided at the first and last row / column; solver.py:396-399), and
+// out[nx + i] = max over y of p + rho |v|^2 / 2 + |B|^2 / 2 (solver.py:412-414).
+// One workgroup per row; FP64 with a fixed summation order.
+template <class real>
+__global__ __launch_bounds__(256) void flow_rows_kernel(const real* __restrict__ u, long plane, int pitch, int nx,
+                                                        int ny, double gamma, double dx, double dy,
+                                                        double* __restrict__ out) {
+    const int i = blockIdx.x;
+    const DiagCell<real> q{u, plane, pitch};
+    const long row = (long)i * pitch;
+    double ens = 0.0, stag = -INFINITY;
+    bool nan = false;
+    for (int j = threadIdx.x; j < ny; j += blockDim.x) {
+        const long o = row + j;
+        double dvydx, dvxdy;
+        if (nx < 2) dvydx = 0.0;
+        else if (i == 0) dvydx = (q.vel(MY, o + pitch) - q.vel(MY, o)) / dx;
+        else if (i == nx - 1) dvydx = (q.vel(MY, o) - q.vel(MY, o - pitch)) / dx;
+        else dvydx = (q.vel(MY, o + pitch) - q.vel(MY, o - pitch)) / (2.0 * dx);
+        if (ny < 2) dvxdy = 0.0;
+        else if (j == 0) dvxdy = (q.vel(MX, o + 1) - q.vel(MX, o)) / dy;
+        else if (j == ny - 1) dvxdy = (q.vel(MX, o) - q.vel(MX, o - 1)) / dy;
+        else dvxdy = (q.vel(MX, o + 1) - q.vel(MX, o - 1)) / (2.0 * dy);
+        const double om = (nx > 1 && ny > 1) ? dvydx - dvxdy : 0.0;
+        const double rho = q.rho(o);
+        ens += rho * (om * om);
+        const double vx = q.vel(MX, o), vy = q.vel(MY, o), vz = q.vel(MZ, o);
+        const double bx = double(u[BX * plane + o]), by = double(u[BY * plane + o]), bz = double(u[BZ * plane + o]);
+        const double ke = 0.5 * rho * (vx * vx + vy * vy + vz * vz);
+        const double me = 0.5 * (bx * bx + by * by + bz * bz);
+        const double p = fmax((gamma - 1.0) * (double(u[EN * plane + o]) - ke - me), 1e-12);
+        const double s = p + 0.5 * rho * (vx * vx + vy * vy) + 0.5 * (bx * bx + by * by + bz * bz);
+        nan = nan || s != s;
+        stag = s > stag ? s : stag;
+    }
+    __shared__ double sh[2][4];
+    __shared__ int shn[4];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    ens = wave_sum(ens);
+    stag = wave_max(stag);
+    const bool wnan = __builtin_amdgcn_ballot_w64(nan) != 0;
+    if (lane == 0) { sh[0][w] = ens; sh[1][w] = stag; shn[w] = wnan; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        out[i] = (sh[0][0] + sh[0][1]) + (sh[0][2] + sh[0][3]);
+        const double m = fmax(fmax(sh[1][0], sh[1][1]), fmax(sh[1][2], sh[1][3]));
+        out[nx + i] = (shn[0] | shn[1] | shn[2] | shn[3]) ? NAN : m;   // np.max propagates NaN
+    }
+}
+
+// Interface x position of each interior column j (solver.py:372-384): with
+// g(i) = |dC/dx| as np.gradient forms it on C = clip(C, 0, 1), if max g >
+// 1e-6, k = the first row holding the maximum, hw = min(10, k, nx - k - 1),
+// and for hw > 0 the g-weighted mean of x over rows k - hw .. k + hw when
+// the weights sum above 1e-12; else NaN.  A column holding a NaN gradient
+// gives NaN (np.max is NaN there).  Workgroup: 64 columns (x of the thread
+// index) by IFACE_CHUNKS row chunks (y), combined in LDS.
+constexpr int IFACE_CHUNKS = 16;
+template <class real>
+__global__ __launch_bounds__(64 * IFACE_CHUNKS) void iface_cols_kernel(const real* __restrict__ u, long plane,
+                                                                       int pitch, int nx, int ny, double dx,
+                                                                       double x_min, double* __restrict__ pos) {
+    const int jl = threadIdx.x, ch = threadIdx.y;
+    const int j = blockIdx.x * 64 + jl;
+    const DiagCell<real> q{u, plane, pitch};
+    auto grad = [&](int i) {
+        const long o = (long)i * pitch + j;
+        double g;
+        if (i == 0) g = (q.conc(o + pitch) - q.conc(o)) / dx;
+        else if (i == nx - 1) g = (q.conc(o) - q.conc(o - pitch)) / dx;
+        else g = (q.conc(o + pitch) - q.conc(o - pitch)) / (2.0 * dx);
+        return fabs(g);
+    };
+    double best = -INFINITY;
+    int bi = 0;
+    bool nan = false;
+    if (j < ny && nx > 1) {
+        const int per = (nx + IFACE_CHUNKS - 1) / IFACE_CHUNKS;
+        const int ia = ch * per, ib = min(nx, ia + per);
+        for (int i = ia; i < ib; ++i) {
+            const double g = grad(i);
+            nan = nan || g != g;
+            if (g > best) { best = g; bi = i; }
+        }
+    }
+    __shared__ double sb[IFACE_CHUNKS][64];
+    __shared__ int si[IFACE_CHUNKS][64];
+    __shared__ int sn[IFACE_CHUNKS][64];
+    sb[ch][jl] = best; si[ch][jl] = bi; sn[ch][jl] = nan;
+    __syncthreads();
+    if (ch != 0 || j >= ny) return;
+    // chunks in row order: a later chunk wins only with a larger value, so
+    // the first row holding the maximum is kept (np.argmax)
+    for (int c = 1; c < IFACE_CHUNKS; ++c) {
+        if (sb[c][jl] > best) { best = sb[c][jl]; bi = si[c][jl]; }
+        nan = nan || sn[c][jl];
+    }
+    double r = NAN;
+    if (!nan && nx > 1 && best > 1e-6) {
+        const int k = bi;
+        const int hw = min(10, min(k, nx - k - 1));
+        if (hw > 0) {
+            double ws = 0.0, xs = 0.0;
+            for (int t = k - hw; t <= k + hw; ++t) {
+                const double g = grad(t);
+                ws += g;
+                xs += (x_min + (t + 0.5) * dx) * g;
+            }
+            if (ws > 1e-12) r = xs / ws;
+        }
+    }
+    pos[j] = r;
+}
+
 }  // namespace
 
 extern "C" {
@@ -894,6 +1025,33 @@ int amhd_concentration_profile(void* ctx, double* mean, double* var) {
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(mean, d, sizeof(double) * c->nx, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipMemcpyAsync(var, d + c->nx, sizeof(double) * c->nx, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int amhd_flow_diagnostics(void* ctx, double* row_ens, double* row_stag, double* col_pos) {
+    if (!ctx || !row_ens || !row_stag || !col_pos) return fail(-1, "amhd_flow_diagnostics: null argument");
+    Ctx* c = (Ctx*)ctx;
+    double* d = c->staging;   // 2 nx + ny doubles of scratch
+    const dim3 ib(64, IFACE_CHUNKS);
+    const int ig = (c->ny + 63) / 64;
+    if (c->dtype == AMHD_F64) {
+        const double* u = (const double*)c->buf[c->cur];
+        flow_rows_kernel<double><<<c->nx, 256, 0, c->stream>>>(u, c->plane, c->pitch, c->nx, c->ny, c->p.gamma,
+                                                               c->p.dx, c->p.dy, d);
+        iface_cols_kernel<double><<<ig, ib, 0, c->stream>>>(u, c->plane, c->pitch, c->nx, c->ny, c->p.dx, c->p.x_min,
+                                                            d + 2 * (long)c->nx);
+    } else {
+        const float* u = (const float*)c->buf[c->cur];
+        flow_rows_kernel<float><<<c->nx, 256, 0, c->stream>>>(u, c->plane, c->pitch, c->nx, c->ny, c->p.gamma,
+                                                              c->p.dx, c->p.dy, d);
+        iface_cols_kernel<float><<<ig, ib, 0, c->stream>>>(u, c->plane, c->pitch, c->nx, c->ny, c->p.dx, c->p.x_min,
+                                                           d + 2 * (long)c->nx);
+    }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(row_ens, d, sizeof(double) * c->nx, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(row_stag, d + c->nx, sizeof(double) * c->nx, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(col_pos, d + 2 * (long)c->nx, sizeof(double) * c->ny, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     return 0;
 }

@@ -167,6 +167,18 @@ class Context:
                  "amhd_concentration_profile")
         return mean, var
 
+    def flow_diagnostics(self):
+        """(row_ens, row_stag, col_pos): per-row sums of rho omega^2, per-row
+        maxima of the stagnation pressure and per-column interface positions
+        (NaN where none), reduced on the device."""
+        ens = np.empty(self.nx, dtype=np.float64)
+        stag = np.empty(self.nx, dtype=np.float64)
+        pos = np.empty(self.ny, dtype=np.float64)
+        dp = C.POINTER(C.c_double)
+        _L.check(self.lib.amhd_flow_diagnostics(self.h, ens.ctypes.data_as(dp), stag.ctypes.data_as(dp),
+                                                pos.ctypes.data_as(dp)), "amhd_flow_diagnostics")
+        return ens, stag, pos
+
 
 class MHDSolver:
     """2-D ideal MHD: MUSCL/van Leer + HLLD, SSP-RK3, GLM + Powell
@@ -356,12 +368,25 @@ class MHDSolver:
                 "energy_drift_corrected_pct": abs(energy - bf - e0) / max(abs(e0), 1e-14) * 100,
                 "boundary_flux": bf}
 
-    def compute_diagnostics(self):
-        """Diagnostics of solver.py:335-444 on the downloaded state."""
-        from amhd_diag import diagnostics
+    def diagnostics_dict(self) -> dict:
+        """Every diagnostic of solver.py:335-444 without downloading the
+        state: row and column reductions in FP64 on the device
+        (amhd_global_sums, amhd_concentration_profile,
+        amhd_flow_diagnostics), then the O(nx + ny) rest (trapezoids, the
+        interface's interpolation and FFT) on the host, with the steps of
+        amhd_diag.diagnostics."""
+        from amhd_diag import from_reductions
 
         self._sync_state()
-        d = diagnostics(self.ctx.download_numpy(), self.cfg)
+        mass, energy, div_max, div_l2 = self.global_sums()
+        cbar, cvar = self.ctx.concentration_profile()
+        row_ens, row_stag, pos = self.ctx.flow_diagnostics()
+        return from_reductions(self.cfg, cbar, cvar, row_ens, row_stag, pos, mass, energy, div_max, div_l2)
+
+    def compute_diagnostics(self):
+        """Diagnostics of solver.py:335-444, appended to the diag_* lists as
+        the reference's loop does; the state stays on the device."""
+        d = self.diagnostics_dict()
         self.diag_times.append(self.t)
         self.diag_mixing_width_integral.append(d["mw_integral"])
         self.diag_mixing_width_thresh.append(d["mw_thresh"])

@@ -153,6 +153,18 @@ int amhd_global_sums(void* ctx, double out[4]);
  * FP64 (host arrays of nx; synchronises). */
 int amhd_concentration_profile(void* ctx, double* mean, double* var);
 
+/* The rest of compute_diagnostics (solver.py:335-444) reduced on the device
+ * in FP64, from cons_to_prim with its 1e-12 floors:
+ * row_ens[i] = sum over y of rho * omega^2 for interior row i, omega =
+ * dvy/dx - dvx/dy with np.gradient's differences (solver.py:396-399);
+ * row_stag[i] = max over y of p + rho |v|^2 / 2 + |B|^2 / 2 (solver.py:
+ * 412-414; NaN if any cell is NaN);
+ * col_pos[j] = the interface x position of column j as solver.py:372-384
+ * finds it (the |dC/dx|-weighted mean of x over the 2 hw + 1 rows around
+ * the first maximum), NaN where it finds none.
+ * Host arrays of nx, nx and ny doubles; synchronises. */
+int amhd_flow_diagnostics(void* ctx, double* row_ens, double* row_stag, double* col_pos);
+
 /* Learned-closure hooks (no counterpart in the reference).
  * amhd_state_check: out = {min rho, min p, max|divB|, non-finite cells}.
  * amhd_state_ptr: device address of the current interior-layout state

@@ -141,6 +141,20 @@ def test_rmi_uniform_bx_full_size_properties(torch_gpu, precision):
     assert Wf[0].min() > 0 and Wf[7].min() > 0
     for k in ("mass_total", "energy_total", "divB_max", "mw_integral", "mixedness"):
         assert d[k] == pytest.approx(h[k], rel=1e-12, abs=1e-300), k
+    # every diagnostic of compute_diagnostics from the device reductions
+    # (amhd_flow_diagnostics for enstrophy, stagnation pressure and the
+    # interface): FP64 sums in another order, so 1e-11 relative
+    dd = s.diagnostics_dict()
+    say(f"  device diagnostics: ens {dd['enstrophy']:.6e} local {dd['enstrophy_local']:.6e} "
+        f"pert {dd['pert_amp']:.4e} stag {dd['stag']:.6e}")
+    assert set(dd) == set(h)
+    for k in h:
+        if k == "mode_amps":
+            assert np.allclose(dd[k], h[k], rtol=1e-10, atol=1e-12 * np.max(h[k])), k
+        else:
+            assert dd[k] == pytest.approx(h[k], rel=1e-11, abs=1e-300), k
+    assert dd["stag"] == h["stag"] and dd["mw_thresh"] == h["mw_thresh"]
+    assert h["pert_amp"] > 0 and h["enstrophy_local"] != h["enstrophy"]
     assert d["energy_drift_raw_pct"] > 1.0
     assert d["energy_drift_corrected_pct"] < 0.01 * d["energy_drift_raw_pct"]
     assert d["mass_total"] > d0["mass_total"]      # inflow through x = 0

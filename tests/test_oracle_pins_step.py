@@ -365,3 +365,49 @@ def test_diag_divb_matches_the_linear_field():
     g = amhd_diag.diagnostics(U, cfg)
     assert g["divB_max"] == pytest.approx(0.5, rel=1e-10)
     assert g["energy_total"] == pytest.approx(float(np.sum(U[7, 2:-2, 2:-2])) * cfg.dx * cfg.dy, rel=1e-14)
+
+
+def _host_reductions(U, cfg, ng=2):
+    """The per-row / per-column reductions amhd_flow_diagnostics,
+    amhd_concentration_profile and amhd_global_sums return, formed on the
+    host from the same definitions."""
+    import amhd_diag
+
+    W = amhd_diag._c2p(U, cfg.gamma)
+    sl = (slice(ng, -ng), slice(ng, -ng))
+    rho, p, vx, vy = W[0][sl], W[7][sl], W[1][sl], W[2][sl]
+    bx, by, bz = W[4][sl], W[5][sl], W[6][sl]
+    cc = np.clip(W[9][sl], 0.0, 1.0)
+    cbar = cc.mean(axis=1)
+    cvar = np.mean((cc - cbar[:, None]) ** 2, axis=1)
+    omega = np.gradient(vy, cfg.dx, axis=0) - np.gradient(vx, cfg.dy, axis=1)
+    row_ens = np.sum(rho * omega ** 2, axis=1)
+    row_stag = np.max(p + 0.5 * rho * (vx ** 2 + vy ** 2) + 0.5 * (bx ** 2 + by ** 2 + bz ** 2), axis=1)
+    x = cfg.x_min + (np.arange(cfg.nx) + 0.5) * cfg.dx
+    pos = amhd_diag.interface_positions(cc, x, cfg.dx)
+    g = amhd_diag.diagnostics(U, cfg)
+    return cbar, cvar, row_ens, row_stag, pos, g["mass_total"], g["energy_total"], g["divB_max"], g["divB_L2"]
+
+
+def test_diag_from_reductions_equals_diagnostics():
+    """The device path of compute_diagnostics (row and column reductions,
+    then amhd_diag.from_reductions on the host) gives what
+    amhd_diag.diagnostics gives on the whole field: a perturbed interface in
+    a rotating flow, so the interface modes, the local enstrophy window and
+    the stagnation maximum are all non-trivial."""
+    import amhd_diag
+
+    cfg = _diag_cfg(nx=200, ny=64)
+    X, Y = _centres(cfg)
+    xif = 2.0 + 0.1 * np.sin(2 * np.pi * cfg.perturbation_mode * Y / (cfg.y_max - cfg.y_min))
+    C = 0.5 * (1.0 + np.tanh((X - xif) / 0.04))
+    U = _state_from_c(cfg, C, rho=1.0 + 2.0 * C, vx=-0.4 * (Y - 0.5), vy=0.3 * np.sin(X))
+    g = amhd_diag.diagnostics(U, cfg)
+    d = amhd_diag.from_reductions(cfg, *_host_reductions(U, cfg))
+    assert set(d) == set(g)
+    assert g["pert_amp"] > 0.05 and g["enstrophy_local"] != g["enstrophy"]
+    for k in g:
+        if k == "mode_amps":
+            assert np.allclose(d[k], g[k], rtol=1e-12, atol=1e-14)
+        else:
+            assert d[k] == pytest.approx(g[k], rel=1e-12), k
