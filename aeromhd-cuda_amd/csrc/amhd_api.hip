@@ -140,10 +140,12 @@ struct Ctx {
     double* part_bf = nullptr;
     double* gsum = nullptr;      // global-sum partials
     StepCtl* ctl = nullptr;
+    StepCtl* ctl_next = nullptr; // the device loop's current step (the fused stage kernels)
     hipStream_t stream = nullptr;
     hipGraphExec_t graph[3] = {nullptr, nullptr, nullptr};
     const void* halo[2] = {nullptr, nullptr};   // x-slab halo buffers (lo, hi), caller-owned
     int* seg_i0 = nullptr;       // device [nx + 1]: first row of each segment, then nx
+    int* seg_next = nullptr;     // device [nx + 1]: the table the device loop's stages 2 and 3 read
     int balance = 0;             // size the segments by cost each step (FP32; amhd_set_balance)
     unsigned char* rowflags = nullptr;   // device [nx][RB_FLAG_PITCH]: rows of uniform x faces
 };
@@ -157,8 +159,8 @@ size_t esize(int dtype) { return dtype == AMHD_F32 ? sizeof(float) : sizeof(doub
 // most the context's limit: 2 by default with two wavefronts per SIMD, else
 // 0) take G of its segments per wavefront, so it costs n_segs / G
 // wavefronts instead of n_segs.  rows == 0: as many segments as fill every
-// resident wavefront slot once, so that every wave of a stage launch runs
-// concurrently with equal work.
+// resident wavefront slot but one workgroup's once, so that every wave of a
+// stage launch runs concurrently with equal work.
 void set_tiling(Ctx* c, int rows) {
     c->rows = rows;
     c->n_strips = (c->ny + STRIP - 1) / STRIP;
@@ -169,15 +171,18 @@ void set_tiling(Ctx* c, int rows) {
     c->pk_groups = g;
     c->pk_shift = g == 8 ? 3 : g == 4 ? 4 : g == 2 ? 5 : 0;
     const long wide = g ? c->n_strips - 1 : c->n_strips;
+    // the device loop's first and third stages add one side workgroup: keep
+    // its slots free so that it runs beside the strips
+    const long slots = std::max(1, c->slots - WPB);
     long segs;
     if (rows > 0) {
         segs = (c->nx + rows - 1) / rows;
     } else if (!g) {
-        segs = c->slots / c->n_strips;
+        segs = slots / c->n_strips;
     } else {
         // largest S with wide * S + ceil(S / g) <= slots
-        segs = (long)c->slots * g / (wide * g + 1);
-        while (segs > 1 && wide * segs + (segs + g - 1) / g > c->slots) --segs;
+        segs = (long)slots * g / (wide * g + 1);
+        while (segs > 1 && wide * segs + (segs + g - 1) / g > slots) --segs;
     }
     c->n_segs = (int)std::max(1L, std::min(segs, (long)c->nx));
 }
@@ -233,6 +238,8 @@ std::vector<int> segment_rows(const Ctx* c) {
 hipError_t upload_segments(Ctx* c) {
     const std::vector<int> i0 = segment_rows(c);
     hipError_t e = hipMemcpyAsync(c->seg_i0, i0.data(), sizeof(int) * i0.size(), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(c->seg_next, i0.data(), sizeof(int) * i0.size(), hipMemcpyHostToDevice, c->stream);
     return e == hipSuccess ? hipStreamSynchronize(c->stream) : e;
 }
 
@@ -282,6 +289,58 @@ hipError_t enqueue_step(Ctx* c, int cur, double dt, double ch, double decay, con
     if ((e = launch_stage_any(c, 1, u0, u0, u1, dt, ch, decay, ctl)) != hipSuccess) return e;
     if ((e = launch_stage_any(c, 2, u0, u1, u2, dt, ch, decay, ctl)) != hipSuccess) return e;
     return launch_stage_any(c, 3, u0, u2, u1, dt, ch, decay, ctl);
+}
+
+// One SSP-RK3 step of the device loop with the step control fused into the
+// stage kernels (no control launch).  Stage 1 reduces the CFL partials of
+// the previous step (or of the state upload) in every workgroup and begins
+// the step from the committed block `ctl` into `ctl_next`; with cost-sized
+// segments its side workgroup re-sizes the next table (seg_next) from the
+// row flags.  Stages 2 and 3 read ctl_next and seg_next; the side workgroup
+// of stage 3 closes the step into `ctl` and copies seg_next to seg_i0 for
+// the next step's first stage.  No table or block is written while a
+// launch reads it.
+// CFL partials the fused third stage leaves: one per strip workgroup.
+int loop_parts(const Ctx* c) { return (n_waves(c) + WPB - 1) / WPB; }
+
+template <class real>
+hipError_t launch_loop_stage(Ctx* c, int stage, const void* u0, const void* us, void* uo) {
+    StageParams<real> a = stage_args<real>(c, stage, (const real*)u0, (const real*)us, (real*)uo);
+    a.fuse = 1;
+    const bool bal = c->balance != 0;
+    if (stage == 1) {
+        a.ctl = c->ctl;
+        a.ctl_out = c->ctl_next;
+        a.part_in = c->part_dt;
+        a.seg_i0 = c->seg_i0;
+        if (bal) {
+            a.side = 1;
+            a.rb = Rebalance{c->rowflags, c->pk_groups ? c->n_strips - 1 : c->n_strips, c->nx, c->n_segs, c->seg_next};
+        }
+    } else {
+        a.ctl = c->ctl_next;
+        a.seg_i0 = bal ? c->seg_next : c->seg_i0;
+        if (stage == 3) {
+            a.side = 1;
+            a.ctl_out = c->ctl;
+            a.part_bf_in = c->p.bc_x_periodic ? nullptr : c->part_bf;
+            a.n_part_next = loop_parts(c);
+            a.seg_copy = bal ? c->seg_i0 : nullptr;
+        }
+    }
+    if constexpr (sizeof(real) == 8) return launch_stage_f64(stage, c->p.powell != 0, true, a, c->stream);
+    else return launch_stage_f32(stage, c->p.powell != 0, true, a, c->stream);
+}
+
+hipError_t enqueue_loop_step(Ctx* c, int cur) {
+    void* u0 = c->buf[cur];
+    void* u1 = c->buf[(cur + 1) % 3];
+    void* u2 = c->buf[(cur + 2) % 3];
+    const bool f64 = c->dtype == AMHD_F64;
+    hipError_t e = f64 ? launch_loop_stage<double>(c, 1, u0, u0, u1) : launch_loop_stage<float>(c, 1, u0, u0, u1);
+    if (e == hipSuccess) e = f64 ? launch_loop_stage<double>(c, 2, u0, u1, u2) : launch_loop_stage<float>(c, 2, u0, u1, u2);
+    if (e == hipSuccess) e = f64 ? launch_loop_stage<double>(c, 3, u0, u2, u1) : launch_loop_stage<float>(c, 3, u0, u2, u1);
+    return e;
 }
 
 hipError_t enqueue_ctl(Ctx* c, int n_part, double host_dt, double cfl, int mode, StepCtl* ctl) {
@@ -578,7 +637,9 @@ int amhd_create(void** out, int nx, int ny, int dtype, const amhd_params* p) {
     if (e == hipSuccess) e = hipMalloc(&c->part_bf, sizeof(double) * 2 * (size_t)c->n_strips);
     if (e == hipSuccess) e = hipMalloc(&c->gsum, sizeof(double) * 4 * GSUM_BLOCKS);
     if (e == hipSuccess) e = hipMalloc(&c->ctl, sizeof(StepCtl));
+    if (e == hipSuccess) e = hipMalloc(&c->ctl_next, sizeof(StepCtl));
     if (e == hipSuccess) e = hipMalloc(&c->seg_i0, sizeof(int) * (size_t)(nx + 1));
+    if (e == hipSuccess) e = hipMalloc(&c->seg_next, sizeof(int) * (size_t)(nx + 1));
     // cost-sized segments where uniform x faces make rows cheaper (FP32)
     c->balance = dtype == AMHD_F32 ? 1 : 0;
     if (e == hipSuccess) e = hipMalloc(&c->rowflags, (size_t)RB_FLAG_PITCH * nx);
@@ -587,6 +648,7 @@ int amhd_create(void** out, int nx, int ny, int dtype, const amhd_params* p) {
     if (e == hipSuccess) {
         StepCtl z{};
         e = hipMemcpyAsync(c->ctl, &z, sizeof z, hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(c->ctl_next, &z, sizeof z, hipMemcpyHostToDevice, c->stream);
         if (e == hipSuccess) e = hipMemsetAsync(c->part_bf, 0, sizeof(double) * 2 * c->n_strips, c->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     }
@@ -610,7 +672,9 @@ int amhd_destroy(void* ctx) {
     if (c->part_bf) hipFree(c->part_bf);
     if (c->gsum) hipFree(c->gsum);
     if (c->ctl) hipFree(c->ctl);
+    if (c->ctl_next) hipFree(c->ctl_next);
     if (c->seg_i0) hipFree(c->seg_i0);
+    if (c->seg_next) hipFree(c->seg_next);
     if (c->rowflags) hipFree(c->rowflags);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
@@ -825,11 +889,11 @@ int amhd_run(void* ctx, int n_steps, double cfl, double t_end, int max_steps, in
     h.min_dxdy = std::min(c->p.dx, c->p.dy);
     h.done = 0;
     h.pending = 0;
-    HIP_TRY(hipMemcpyAsync(c->ctl, &h, sizeof h, hipMemcpyHostToDevice, c->stream));
     int rc = ensure_partials(c);
     if (rc) return rc;
-    HIP_TRY(enqueue_ctl(c, c->n_part, 0.0, cfl, CTL_DEVICE_LOOP, c->ctl));
-    const int nw = n_waves(c);
+    h.n_part = c->n_part;   // the first stage reduces these partials
+    HIP_TRY(hipMemcpyAsync(c->ctl, &h, sizeof h, hipMemcpyHostToDevice, c->stream));
+    const int nw = loop_parts(c);
     if (use_graph && n_steps > 0) {
 
         hipGraphExec_t& ge = c->graph[c->cur];
@@ -837,10 +901,7 @@ int amhd_run(void* ctx, int n_steps, double cfl, double t_end, int max_steps, in
             hipGraph_t g;
             HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
             hipError_t e = hipSuccess;
-            for (int s = 0; s < 3 && e == hipSuccess; ++s) {
-                e = enqueue_step(c, (c->cur + s) % 3, 0.0, 0.0, 1.0, c->ctl);
-                if (e == hipSuccess) e = enqueue_ctl(c, nw, 0.0, cfl, CTL_DEVICE_LOOP, c->ctl);
-            }
+            for (int s = 0; s < 3 && e == hipSuccess; ++s) e = enqueue_loop_step(c, (c->cur + s) % 3);
             hipError_t e2 = hipStreamEndCapture(c->stream, &g);
             HIP_TRY(e);
             HIP_TRY(e2);
@@ -849,10 +910,7 @@ int amhd_run(void* ctx, int n_steps, double cfl, double t_end, int max_steps, in
         }
         for (int r = 0; r < (n_steps + 2) / 3; ++r) HIP_TRY(hipGraphLaunch(ge, c->stream));
     } else {
-        for (int s = 0; s < n_steps; ++s) {
-            HIP_TRY(enqueue_step(c, (c->cur + s) % 3, 0.0, 0.0, 1.0, c->ctl));
-            HIP_TRY(enqueue_ctl(c, nw, 0.0, cfl, CTL_DEVICE_LOOP, c->ctl));
-        }
+        for (int s = 0; s < n_steps; ++s) HIP_TRY(enqueue_loop_step(c, (c->cur + s) % 3));
     }
     HIP_TRY(hipMemcpyAsync(&h, c->ctl, sizeof h, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
@@ -881,13 +939,14 @@ int amhd_run_profiled(void* ctx, int n_steps, double cfl, double t_end, int max_
     h.max_steps = (int)std::min<long>((long)max_steps, (long)start + n_steps);
     h.glm_alpha = c->p.glm_alpha; h.min_dxdy = std::min(c->p.dx, c->p.dy);
     h.done = 0; h.pending = 0;
-    HIP_TRY(hipMemcpyAsync(c->ctl, &h, sizeof h, hipMemcpyHostToDevice, c->stream));
     int rc = ensure_partials(c);
     if (rc) return rc;
-    HIP_TRY(enqueue_ctl(c, c->n_part, 0.0, cfl, CTL_DEVICE_LOOP, c->ctl));
-    const int nw = n_waves(c);
-    // two events around each of the 4 launches of a step
-    std::vector<hipEvent_t> ev((size_t)n_steps * 8);
+    h.n_part = c->n_part;
+    HIP_TRY(hipMemcpyAsync(c->ctl, &h, sizeof h, hipMemcpyHostToDevice, c->stream));
+    const int nw = loop_parts(c);
+    // two events around each of the 3 launches of a step (the stage kernels
+    // carry the step control)
+    std::vector<hipEvent_t> ev((size_t)n_steps * 6);
     for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
     hipError_t err = hipSuccess;
     for (int st = 0; st < n_steps && err == hipSuccess; ++st) {
@@ -895,27 +954,28 @@ int amhd_run_profiled(void* ctx, int n_steps, double cfl, double t_end, int max_
         void* u0 = c->buf[cur];
         void* u1 = c->buf[(cur + 1) % 3];
         void* u2 = c->buf[(cur + 2) % 3];
-        hipEvent_t* e = &ev[(size_t)st * 8];
+        hipEvent_t* e = &ev[(size_t)st * 6];
+        const bool f64 = c->dtype == AMHD_F64;
         if (err == hipSuccess) err = hipEventRecord(e[0], c->stream);
-        if (err == hipSuccess) err = launch_stage_any(c, 1, u0, u0, u1, 0.0, 0.0, 1.0, c->ctl);
+        if (err == hipSuccess)
+            err = f64 ? launch_loop_stage<double>(c, 1, u0, u0, u1) : launch_loop_stage<float>(c, 1, u0, u0, u1);
         if (err == hipSuccess) err = hipEventRecord(e[1], c->stream);
         if (err == hipSuccess) err = hipEventRecord(e[2], c->stream);
-        if (err == hipSuccess) err = launch_stage_any(c, 2, u0, u1, u2, 0.0, 0.0, 1.0, c->ctl);
+        if (err == hipSuccess)
+            err = f64 ? launch_loop_stage<double>(c, 2, u0, u1, u2) : launch_loop_stage<float>(c, 2, u0, u1, u2);
         if (err == hipSuccess) err = hipEventRecord(e[3], c->stream);
         if (err == hipSuccess) err = hipEventRecord(e[4], c->stream);
-        if (err == hipSuccess) err = launch_stage_any(c, 3, u0, u2, u1, 0.0, 0.0, 1.0, c->ctl);
+        if (err == hipSuccess)
+            err = f64 ? launch_loop_stage<double>(c, 3, u0, u2, u1) : launch_loop_stage<float>(c, 3, u0, u2, u1);
         if (err == hipSuccess) err = hipEventRecord(e[5], c->stream);
-        if (err == hipSuccess) err = hipEventRecord(e[6], c->stream);
-        if (err == hipSuccess) err = enqueue_ctl(c, nw, 0.0, cfl, CTL_DEVICE_LOOP, c->ctl);
-        if (err == hipSuccess) err = hipEventRecord(e[7], c->stream);
     }
     if (err == hipSuccess) err = hipMemcpyAsync(&h, c->ctl, sizeof h, hipMemcpyDeviceToHost, c->stream);
     if (err == hipSuccess) err = hipStreamSynchronize(c->stream);
-    double acc[4] = {0, 0, 0, 0};
+    double acc[3] = {0, 0, 0};
     for (int st = 0; st < n_steps && err == hipSuccess; ++st) {
-        for (int k = 0; k < 4 && err == hipSuccess; ++k) {
+        for (int k = 0; k < 3 && err == hipSuccess; ++k) {
             float ms = 0.f;
-            err = hipEventElapsedTime(&ms, ev[(size_t)st * 8 + 2 * k], ev[(size_t)st * 8 + 2 * k + 1]);
+            err = hipEventElapsedTime(&ms, ev[(size_t)st * 6 + 2 * k], ev[(size_t)st * 6 + 2 * k + 1]);
             acc[k] += ms;
         }
     }
@@ -928,7 +988,7 @@ int amhd_run_profiled(void* ctx, int n_steps, double cfl, double t_end, int max_
     *step_inout = h.steps;
     if (last_dt) *last_dt = h.dt;
     for (int k = 0; k < 3; ++k) stage_ms[k] = acc[k];
-    if (ctl_ms) *ctl_ms = acc[3];
+    if (ctl_ms) *ctl_ms = 0.0;   // no step-control launch in the device loop
     return 0;
 }
 

@@ -51,7 +51,7 @@ hipError_t launch_hlld_f32(const float* wl, const float* wr, float* f, float* sm
 
 #ifdef AMHD_WAVE_TIMES
 // Diagnostic builds only: copies the FP32 stage kernels' wave times
-// (4 x 8192 x 2 unsigned 64-bit values) to host memory.  Not part of the
+// (4 x 8192 x 5 unsigned 64-bit values) to host memory.  Not part of the
 // ABI of include/aeromhd.h.
 extern "C" int amhd_debug_wave_times(unsigned long long* out) {
     return hipMemcpyFromSymbol(out, HIP_SYMBOL(amhd::amhd_wave_times), sizeof(amhd::amhd_wave_times)) == hipSuccess

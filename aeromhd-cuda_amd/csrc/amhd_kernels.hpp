@@ -38,11 +38,21 @@ constexpr int WPB = 4;                  // wavefronts per workgroup
 constexpr int NPART_DT = 4;             // {max cf, max(|vx|+|vy|), max|vx|, max|vy|}
 
 #ifdef AMHD_WAVE_TIMES
-// Diagnostic builds only (-DAMHD_WAVE_TIMES): start and end time of every
-// stage-kernel wavefront of the latest launch of each stage, in ticks of
-// the 100 MHz s_memrealtime clock, [stage][wave][start, end].
+// Diagnostic builds only (-DAMHD_WAVE_TIMES): times of every stage-kernel
+// wavefront of the latest launch of each stage, in ticks of the 100 MHz
+// s_memrealtime clock, [stage][wave][start, end of the segment prologue,
+// end of the first row, end], and its hardware place [5] = XCC_ID << 32 |
+// HW_ID (wave slot, SIMD, CU, shader array, shader engine).
 constexpr int WT_WAVES = 8192;
-__device__ unsigned long long amhd_wave_times[4][WT_WAVES][2];
+constexpr int WT_FIELDS = 5;
+__device__ unsigned long long amhd_wave_times[4][WT_WAVES][WT_FIELDS];
+#define AMHD_WT_MARK(STG, W, K)                                                       \
+    do {                                                                              \
+        const unsigned long long t_ = __builtin_amdgcn_s_memrealtime();               \
+        if (lane == 0 && (W) >= 0 && (W) < WT_WAVES) amhd_wave_times[STG][W][K] = t_; \
+    } while (0)
+#else
+#define AMHD_WT_MARK(STG, W, K) do { } while (0)
 #endif
 
 // Device-resident step control for the host-free time loop.
@@ -53,6 +63,56 @@ struct StepCtl {
     double cfmax, vmax;
     int steps, max_steps, done, pending;
     int cur, n_part;   // batched solver: current state buffer, CFL partial count
+};
+
+// The two halves of the device loop's step control, shared by the step
+// control kernel and the fused stage kernels (contraction off: both
+// translation units round them alike).
+// The end of a step that ran: boundary energy (solver.py:316-317), t and
+// step advance (solver.py:481-482).
+__device__ __forceinline__ void ctl_close(StepCtl& h, double bflux) {
+#pragma clang fp contract(off)
+    h.bflux = bflux;
+    h.bflux_cum += bflux * h.dt;
+    h.t += h.dt;
+    h.steps += 1;
+    h.cur = (h.cur + 1) % 3;
+    h.pending = 0;
+}
+// (ch, dt) from the four CFL maxima, exactly as solver.py:206-223 (all
+// reductions are maxima; since ch >= 1.5 max cf, max(cf, ch) == ch in every
+// cell and max_k (|v_k| + ch)/dx == (max_k |v_k| + ch)/dx, so the four maxima
+// determine dt bit for bit); records ch and the maxima in h.
+__device__ __forceinline__ double ctl_dt(StepCtl& h, const double (&r)[4], double dx, double dy, double cfl) {
+#pragma clang fp contract(off)
+    const double cfmax = sqrt(r[0]);   // partials hold max cf^2
+    const double ch = fmax(cfmax + r[1], 1.0) * 1.5;
+    const double sx = r[2] + ch, sy = r[3] + ch;
+    const double sm = fmax(sx / dx, sy / dy);
+    const double dt = (sm < 1e-14) ? cfl * fmin(dx, dy) : cfl / sm;
+    h.cfmax = cfmax;
+    h.vmax = r[1];
+    h.ch = ch;
+    return dt;
+}
+// The loop test of solver.py:475-479 and the step's dt (clipped to t_end)
+// and psi decay; pending marks a step that is to run.
+__device__ __forceinline__ void ctl_begin(StepCtl& h, double dt) {
+#pragma clang fp contract(off)
+    if (!(h.t < h.t_end && h.steps < h.max_steps)) { h.done = 1; return; }
+    const double rem = h.t_end - h.t;
+    if (rem < dt) dt = rem;
+    if (dt <= 1e-16) { h.done = 1; return; }
+    h.dt = dt;
+    h.decay = h.ch > 0.0 ? exp(-h.glm_alpha * h.ch * dt / h.min_dxdy) : 1.0;
+    h.pending = 1;
+}
+
+// Segment re-sizing (rebalance_block below): row flags in, boundary table out.
+struct Rebalance {
+    const unsigned char* flags;   // [nx][RB_FLAG_PITCH]
+    int n_wide, nx, n_segs;
+    int* seg_i0;                  // [n_segs + 1]
 };
 
 template <class real> struct StageParams {
@@ -88,6 +148,26 @@ template <class real> struct StageParams {
     // (null = none); the step control sizes the next step's segments from
     // them
     unsigned char* rowflags;
+    // Fused device loop (fuse = 1): STAGE 1 reduces the previous step's CFL
+    // partials (part_in, ctl->n_part of them) in every workgroup, reads the
+    // committed control block `ctl` and writes the step's block to ctl_out
+    // (workgroup 0); STAGE 2 and 3 read the step's block through `ctl`.
+    // side = 1: one workgroup after the strips does side work instead:
+    // STAGE 1 re-sizes the segments into rb.seg_i0 (a copy of seg_i0 where
+    // it cannot), STAGE 3 closes the step from `ctl` and part_bf_in into
+    // ctl_out (the committed block) and copies seg_i0 to seg_copy.
+    int fuse, side;
+    const double* part_in;
+    StepCtl* ctl_out;
+    const double* part_bf_in;
+    int n_part_next;   // STAGE 3 side: the CFL partials the stage writes (one per strip workgroup)
+    Rebalance rb;
+    int* seg_copy;
+};
+
+// Step values a fused stage kernel hands to its wavefronts.
+struct CtlVals {
+    double dt, ch, decay;
 };
 
 // Row flags of the cost-sized segments: one 16-byte-aligned record per row
@@ -128,11 +208,12 @@ __device__ __forceinline__ void bstore(double x, __amdgpu_buffer_rsrc_t r, unsig
 // XCD-aware workgroup order: the hardware deals workgroups round-robin over
 // the 8 XCDs; renumber so that consecutive logical workgroups (neighbouring
 // strips, which share halo columns) land on the same XCD and its L2.
-__device__ __forceinline__ int xcd_logical_block() {
-    const int b = blockIdx.x, nb = gridDim.x;
+__device__ __forceinline__ int xcd_logical_block(int nb) {
+    const int b = blockIdx.x;
     const int nb8 = nb & ~7;
     return b < nb8 ? (b & 7) * (nb8 >> 3) + (b >> 3) : b;
 }
+__device__ __forceinline__ int xcd_logical_block() { return xcd_logical_block((int)gridDim.x); }
 
 // Floors of the MUSCL face states (physics.py:329-336).  The exact policy
 // keeps the reference's comparison (a NaN passes through); the fast one
@@ -202,6 +283,183 @@ __device__ __forceinline__ double bflux_term(const real (&u)[NV], double gamma, 
     return ((ud[EN] + pt) * w[iVX] - w[iBX] * vb) * dy;
 }
 
+// Segment boundaries sized by cost.  A row whose x faces all took the
+// uniform-face path costs about 5/7 of one that solved them (a least-squares
+// fit of per-wavefront durations on the magnetic sieve, 18000 wavefronts at
+// steps 200, 800 and 1500: 1.9-2.0 us per uniform row against 2.66-2.74 us
+// per solved row, plus 5-8 us per wavefront), so row r of the wide strips
+// costs sum over strips of (5 or 7), and segment g (0 < g < n_segs) starts
+// after the row whose cost interval (P(r), P(r + 1)] holds the target
+// g total / n_segs.  With at least two rows per segment the targets lie
+// more than one row's cost apart (a row costs at most 7/5 of the mean row,
+// and 2 x 5 > 7), so the boundaries increase strictly and every segment
+// keeps at least one row.  One workgroup; larger grids than the LDS limits
+// below, or fewer than two rows per segment, keep the table as it is (returns false).
+constexpr int RB_MAX_ROWS = 8192;
+#ifndef AMHD_RB_W_UNIFORM
+#define AMHD_RB_W_UNIFORM 5
+#define AMHD_RB_W_SOLVED 7
+#endif
+constexpr int RB_W_UNIFORM = AMHD_RB_W_UNIFORM, RB_W_SOLVED = AMHD_RB_W_SOLVED;
+__device__ __forceinline__ bool rebalance_block(const Rebalance& rb) {
+    __shared__ int rcost[RB_MAX_ROWS];
+    __shared__ long wtot[4];
+    const int T = (int)blockDim.x, t = (int)threadIdx.x;
+    const int nx = rb.nx, ns = rb.n_wide, ng = rb.n_segs;
+    if (nx > RB_MAX_ROWS || nx < 2 * ng || ng < 2 || ns < 1 || ns > RB_FLAG_PITCH || T != 256) return false;
+    // each thread's run of rows [r0, r1)
+    const int R = (nx + T - 1) / T;
+    const int r0 = t * R < nx ? t * R : nx, r1 = r0 + R < nx ? r0 + R : nx;
+    // row costs: the flag bytes (0 or 1, padding 0) of a row counted by
+    // popcount, 16 bytes per load, four rows' loads in flight at a time
+    const int words = (ns + 15) / 16;
+    long loc = 0;
+    for (int rb0 = r0; rb0 < r1; rb0 += 4) {
+        uint4 v[4][RB_FLAG_PITCH / 16];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+#pragma unroll
+            for (int w = 0; w < RB_FLAG_PITCH / 16; ++w) {
+                const bool ok = rb0 + u < r1 && w < words;
+                v[u][w] = ok ? *(const uint4*)(rb.flags + (long)(rb0 + u) * RB_FLAG_PITCH + 16 * w) : make_uint4(0, 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            int uni = 0;
+#pragma unroll
+            for (int w = 0; w < RB_FLAG_PITCH / 16; ++w)
+                uni += __popc(v[u][w].x) + __popc(v[u][w].y) + __popc(v[u][w].z) + __popc(v[u][w].w);
+            if (rb0 + u < r1) {
+                const int c = uni * RB_W_UNIFORM + (ns - uni) * RB_W_SOLVED;
+                rcost[rb0 + u] = c;
+                loc += c;
+            }
+        }
+    }
+    // exclusive prefix of the runs: inclusive scan in each wavefront, then
+    // the wavefront totals
+    const int lane = t & 63, wv = t >> 6;
+    long inc = loc;
+    for (int d = 1; d < 64; d <<= 1) {
+        const long o = __shfl_up(inc, d, 64);
+        if (lane >= d) inc += o;
+    }
+    if (lane == 63) wtot[wv] = inc;
+    __syncthreads();
+    long P = inc - loc, total = 0;
+    for (int u = 0; u < 4; ++u) {
+        if (u < wv) P += wtot[u];
+        total += wtot[u];
+    }
+    for (int r = r0; r < r1; ++r) {
+        const long Pn = P + rcost[r];
+        for (long g = P * ng / total + 1; g <= Pn * ng / total && g < ng; ++g) rb.seg_i0[g] = r + 1;
+        P = Pn;
+    }
+    if (t == 0) {
+        rb.seg_i0[0] = 0;
+        rb.seg_i0[ng] = nx;
+    }
+    return true;
+}
+
+// Net boundary energy flux of a step from the per-strip sums of its first
+// stage (one wavefront, a fixed order; valid in every lane): the left
+// boundary's sum minus the right one's.
+__device__ __forceinline__ double bflux_wave(const double* part_bf, int n_bf, int lane) {
+#pragma clang fp contract(off)
+    double b0 = 0.0, b1 = 0.0;
+    for (int k = lane; k < n_bf; k += 64) { b0 += part_bf[2 * k]; b1 += part_bf[2 * k + 1]; }
+    b0 = wave_sum(b0);
+    b1 = wave_sum(b1);
+    return b0 - b1;
+}
+
+// The CFL maxima of the wavefronts of a workgroup (fused third stage).
+__device__ __forceinline__ double (&block_red())[WPB][NPART_DT] {
+    __shared__ double r[WPB][NPART_DT];
+    return r;
+}
+
+// A double known to be equal in every lane, moved to scalar registers.
+__device__ __forceinline__ double uniform_d(double x) {
+    const int lo = __builtin_amdgcn_readfirstlane(__double2loint(x));
+    const int hi = __builtin_amdgcn_readfirstlane(__double2hiint(x));
+    return __hiloint2double(hi, lo);
+}
+
+// Fused device loop, STAGE 1: every wavefront reduces the previous step's
+// CFL partials itself and begins the step from the committed control block
+// the way the step control kernel does (the same maxima and expressions, so
+// every wavefront holds the same values); wavefront 0 of workgroup 0 stores
+// the step's block.  First half: this lane's maxima over partials lane,
+// lane + 64, ... (eight in flight), issued before the segment's first rows
+// so that both latencies overlap.
+__device__ __forceinline__ void lane_cfl_maxima(const double* __restrict__ part, int n_part, int lane,
+                                                double (&m)[NPART_DT]) {
+    constexpr int U = 8;
+#pragma unroll
+    for (int q = 0; q < NPART_DT; ++q) m[q] = 0.0;
+    for (int k0 = lane; k0 < n_part; k0 += WAVE * U) {
+        double v[U][NPART_DT];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int k = k0 + u * WAVE;
+#pragma unroll
+            for (int q = 0; q < NPART_DT; ++q) v[u][q] = k < n_part ? part[(long)k * NPART_DT + q] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+#pragma unroll
+            for (int q = 0; q < NPART_DT; ++q) m[q] = fmax(m[q], v[u][q]);
+        }
+    }
+}
+// Second half: the wave's maxima and the step values.  Returns false when
+// the loop is done (no step runs).
+template <class real>
+__device__ __forceinline__ bool fused_begin(const StageParams<real>& a, const double (&m)[NPART_DT], CtlVals& cv) {
+    double r[NPART_DT];
+#pragma unroll
+    for (int q = 0; q < NPART_DT; ++q) r[q] = wave_max(m[q]);
+    StepCtl h = *a.ctl;
+    if (!h.done) ctl_begin(h, ctl_dt(h, r, a.dx, a.dy, h.cfl));
+    if (blockIdx.x == 0 && threadIdx.x == 0) *a.ctl_out = h;
+    cv.dt = uniform_d(h.dt);
+    cv.ch = uniform_d(h.ch);
+    cv.decay = uniform_d(h.decay);
+    return !h.done;
+}
+
+// The side workgroup of a fused launch (see StageParams::side).
+template <class real, int STAGE>
+__device__ __forceinline__ void stage_side(const StageParams<real>& a) {
+    if constexpr (STAGE == 1) {
+        // the next segment table from the row flags of the previous step's
+        // third stage; a copy of the current one where it cannot be re-sized
+        if (!rebalance_block(a.rb)) {
+            for (int k = threadIdx.x; k <= a.n_segs; k += blockDim.x) a.rb.seg_i0[k] = a.seg_i0[k];
+        }
+    } else if constexpr (STAGE == 3) {
+        // every stage of the step has read the step's block: close the step
+        if (threadIdx.x < 64) {
+            const double bflux = a.part_bf_in ? bflux_wave(a.part_bf_in, a.n_strips, (int)threadIdx.x) : 0.0;
+            if (threadIdx.x == 0) {
+                StepCtl h = *a.ctl;
+                if (h.pending) {
+                    ctl_close(h, bflux);
+                    h.n_part = a.n_part_next;
+                }
+                *a.ctl_out = h;
+            }
+        }
+        if (a.seg_copy) {
+            for (int k = threadIdx.x; k <= a.n_segs; k += blockDim.x) a.seg_copy[k] = a.seg_i0[k];
+        }
+    }
+}
+
 // ------------------------------------------------------------ stage kernel
 // STAGE 0: R = L(U^s) only (compute_rhs, solver.py:268-308)
 // STAGE 1: U1 = U0 + dt L(U0)                              (solver.py:319)
@@ -245,22 +503,32 @@ __device__ __forceinline__ void stage_wave(const StageParams<real>& a, const int
     int n_rows = i1 - i0;
     if constexpr (PK) n_rows = wave_max_int(live ? n_rows : 0);
 
+    // The step's dt, ch and psi decay: launch arguments, the device loop's
+    // control block, or (fused first stage) reduced by the wavefront beside
+    // the segment's first rows (lane_cfl_maxima, fused_begin in the prologue)
+    const bool fused = STAGE == 1 && a.fuse;
     double dt_d = a.dt, ch_d = a.ch, decay_d = a.decay;
-    if (a.ctl) {
+    if (a.ctl && !fused) {
         if (a.ctl->done) return;
         dt_d = a.ctl->dt; ch_d = a.ctl->ch; decay_d = a.ctl->decay;
     }
-    const FluxConst<real> kc = make_flux_const<real>(a.gamma, ch_d);
+    FluxConst<real> kc;
     // uniform x faces take the physical flux (FP32 with GLM only: the FP64
     // path keeps the reference's arithmetic bit for bit)
     constexpr bool XSHORT = !P::exact && GLM;
-    const real dtr = real(dt_d);
+    real dtr, cx, cy, pcx, pcy;
     const real dxr = real(a.dx), dyr = real(a.dy);
     const real idx = real(1.0 / a.dx), idy = real(1.0 / a.dy);
     // fast path: weight of dt L(U) in the stage's combine, times 1/dx, 1/dy, 1
     constexpr double wst = STAGE == 1 ? 1.0 : (STAGE == 2 ? 0.25 : 2.0 / 3.0);
-    const real cx = real(wst * dt_d / a.dx), cy = real(wst * dt_d / a.dy);
-    const real pcx = real(0.5 * wst * dt_d / a.dx), pcy = real(0.5 * wst * dt_d / a.dy);
+    auto set_step = [&](double dt, double ch, double decay) {
+        dt_d = dt; ch_d = ch; decay_d = decay;
+        kc = make_flux_const<real>(a.gamma, ch_d);
+        dtr = real(dt_d);
+        cx = real(wst * dt_d / a.dx); cy = real(wst * dt_d / a.dy);
+        pcx = real(0.5 * wst * dt_d / a.dx); pcy = real(0.5 * wst * dt_d / a.dy);
+    };
+    set_step(dt_d, ch_d, decay_d);
 
     constexpr unsigned ES = sizeof(real);
     const unsigned pb = (unsigned)a.plane * ES;             // plane stride, bytes
@@ -329,11 +597,24 @@ __device__ __forceinline__ void stage_wave(const StageParams<real>& a, const int
     real bxm;               // Bx of row i-1 (Powell)
     {
         real wm2[NV], wm1[NV], fa[NV], fb[NV], dm0[NV];
-        prim_row(i0 - 2, wm2);
-        prim_row(i0 - 1, wm1);
+        real um2[NV], um1[NV];
+        // fused first stage: the CFL partials (cache-resident) are loaded
+        // first and reduced while the segment's rows, loaded next, are on
+        // their way from memory
+        double pm[NPART_DT];
+        if (fused) lane_cfl_maxima(a.part_in, a.ctl->n_part, lane, pm);
+        load_in(i0 - 2, um2);
+        load_in(i0 - 1, um1);
         load_in(i0, QA);
-        cons_to_prim<real, P>(QA, A, kc.gamma, psi_ghost(i0));
         load_in(i0 + 1, QB);
+        if (fused) {
+            CtlVals v;
+            if (!fused_begin(a, pm, v)) return;   // the loop is done
+            set_step(v.dt, v.ch, v.decay);
+        }
+        cons_to_prim<real, P>(um2, wm2, kc.gamma, psi_ghost(i0 - 2));
+        cons_to_prim<real, P>(um1, wm1, kc.gamma, psi_ghost(i0 - 1));
+        cons_to_prim<real, P>(QA, A, kc.gamma, psi_ghost(i0));
         cons_to_prim<real, P>(QB, B, kc.gamma, psi_ghost(i0 + 1));
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
@@ -371,6 +652,7 @@ __device__ __forceinline__ void stage_wave(const StageParams<real>& a, const int
         }
         bxm = wm1[iBX];
     }
+    if constexpr (!PK) AMHD_WT_MARK(STAGE, wave, 1);
 
     real mx[NPART_DT] = {0, 0, 0, 0};
     double bf_sum[2] = {0.0, 0.0};
@@ -587,6 +869,7 @@ __device__ __forceinline__ void stage_wave(const StageParams<real>& a, const int
         if (t >= n_rows) break;
         row(i0 + t, A, B, C, QA, QC, F0, F1);
         progress(++t);
+        if (!PK && t == 1) AMHD_WT_MARK(STAGE, wave, 2);
         if (t >= n_rows) break;
         row(i0 + t, B, C, A, QB, QA, F1, F2);
         progress(++t);
@@ -596,14 +879,18 @@ __device__ __forceinline__ void stage_wave(const StageParams<real>& a, const int
     }
 
     if constexpr (STAGE == 3) {
-        if (a.part_dt) {
+        if (a.part_dt || a.fuse) {
             double r[NPART_DT];
 #pragma unroll
             for (int q = 0; q < NPART_DT; ++q) r[q] = wave_max(double(mx[q]));
             const int slot = part >= 0 ? part : wave;
             if (lane == 0) {
 #pragma unroll
-                for (int q = 0; q < NPART_DT; ++q) a.part_dt[(long)slot * NPART_DT + q] = r[q];
+                for (int q = 0; q < NPART_DT; ++q) {
+                    // fused loop: one partial per workgroup, combined by the caller
+                    if (a.fuse) block_red()[threadIdx.x >> 6][q] = r[q];
+                    else a.part_dt[(long)slot * NPART_DT + q] = r[q];
+                }
             }
         }
     }
@@ -625,24 +912,46 @@ __device__ __forceinline__ void stage_wave(const StageParams<real>& a, const int
 template <class real, class P, int STAGE, bool POWELL, bool GLM>
 __global__ __launch_bounds__(WAVE* WPB) AMHD_STAGE_OCC void stage_kernel(StageParams<real> a) {
     const int lane = threadIdx.x & (WAVE - 1);
+    const int nb = (int)gridDim.x - a.side;   // the strip workgroups
+    if ((int)blockIdx.x >= nb) {
+        if constexpr (STAGE == 1 || STAGE == 3) stage_side<real, STAGE>(a);
+        return;
+    }
     // wave-uniform quantities live in scalar registers
 #ifdef AMHD_WAVE_TIMES
     const unsigned long long t_start = __builtin_amdgcn_s_memrealtime();
 #endif
-    const int wave = __builtin_amdgcn_readfirstlane(xcd_logical_block() * WPB + (int)(threadIdx.x >> 6));
+    const int wave = __builtin_amdgcn_readfirstlane(xcd_logical_block(nb) * WPB + (int)(threadIdx.x >> 6));
     const bool older = (int)blockIdx.x < a.first_blocks;
     const int n_wide = (a.pk_groups ? a.n_strips - 1 : a.n_strips) * a.n_segs;
     if (wave < n_wide) {
         stage_wave<real, P, STAGE, POWELL, GLM>(a, wave, lane, older);
-    } else if constexpr (STAGE >= 1) {
-        if (a.pk_groups && wave < stage_wave_count(a))
-            stage_wave<real, P, STAGE, POWELL, GLM, true>(a, wave - n_wide, lane, older, wave);
+    } else if (STAGE >= 1 && a.pk_groups && wave < stage_wave_count(a)) {
+        if constexpr (STAGE >= 1) stage_wave<real, P, STAGE, POWELL, GLM, true>(a, wave - n_wide, lane, older, wave);
+    } else if (STAGE == 3 && a.fuse && lane == 0) {
+#pragma unroll
+        for (int q = 0; q < NPART_DT; ++q) block_red()[threadIdx.x >> 6][q] = 0.0;
+    }
+    if constexpr (STAGE == 3) {
+        // fused loop: the workgroup's CFL partial, the maxima over its
+        // wavefronts (none when the loop is done: the last step's stay)
+        if (a.fuse) {
+            __syncthreads();
+            if (threadIdx.x < NPART_DT && !a.ctl->done) {
+                double m = block_red()[0][threadIdx.x];
+                for (int w = 1; w < WPB; ++w) m = fmax(m, block_red()[w][threadIdx.x]);
+                a.part_dt[(long)xcd_logical_block(nb) * NPART_DT + threadIdx.x] = m;
+            }
+        }
     }
 #ifdef AMHD_WAVE_TIMES
     const unsigned long long t_end = __builtin_amdgcn_s_memrealtime();
     if (lane == 0 && wave < WT_WAVES) {
         amhd_wave_times[STAGE][wave][0] = t_start;
-        amhd_wave_times[STAGE][wave][1] = t_end;
+        amhd_wave_times[STAGE][wave][3] = t_end;
+        const unsigned hw = __builtin_amdgcn_s_getreg((31 << 11) | 4);
+        const unsigned xcc = __builtin_amdgcn_s_getreg((31 << 11) | 20);
+        amhd_wave_times[STAGE][wave][4] = ((unsigned long long)xcc << 32) | hw;
     }
 #endif
 }
@@ -721,11 +1030,8 @@ __global__ __launch_bounds__(256) void dt_partials_w_kernel(const double* __rest
 }
 
 
-// Reduces CFL partials to (ch, dt) exactly as solver.py:206-223 (all
-// reductions are maxima; since ch >= 1.5 max cf, max(cf, ch) == ch in every
-// cell and max_k (|v_k| + ch)/dx == (max_k |v_k| + ch)/dx, so the four maxima
-// determine dt bit for bit).  Also closes a step: boundary-flux sum and its
-// accumulation (solver.py:316-317), t and step advance (solver.py:481-482).
+// Reduces CFL partials to (ch, dt) (ctl_dt) and, in the device loop, also
+// closes the previous step (ctl_close) and begins the next (ctl_begin).
 // One workgroup of 256 threads.  CTL_DEVICE_LOOP reads cfl from ctl.
 // The step-control update of step_ctl_block on a register copy of StepCtl.
 __device__ __forceinline__ void step_ctl_update(StepCtl& h, const double (&r)[NPART_DT], double bflux, double dx,
@@ -735,41 +1041,18 @@ __device__ __forceinline__ void step_ctl_update(StepCtl& h, const double (&r)[NP
         h.bflux_cum += bflux * host_dt;
         return;
     }
-    if (mode == CTL_DEVICE_LOOP && h.pending) {
-        h.bflux = bflux;
-        h.bflux_cum += bflux * h.dt;
-        h.t += h.dt;
-        h.steps += 1;
-        h.cur = (h.cur + 1) % 3;
-        h.pending = 0;
-    }
+    if (mode == CTL_DEVICE_LOOP && h.pending) ctl_close(h, bflux);
     if (mode == CTL_DEVICE_LOOP) {
         if (h.done) return;
         cfl = h.cfl;
     }
-
-    const double cfmax = sqrt(r[0]);   // partials hold max cf^2
-    const double ch = fmax(cfmax + r[1], 1.0) * 1.5;
-    const double sx = r[2] + ch, sy = r[3] + ch;
-    const double sm = fmax(sx / dx, sy / dy);
-    double dt = (sm < 1e-14) ? cfl * fmin(dx, dy) : cfl / sm;
-    h.cfmax = cfmax;
-    h.vmax = r[1];
-    h.ch = ch;
+    const double dt = ctl_dt(h, r, dx, dy, cfl);
     if (mode == CTL_DT_ONLY) {
         h.dt = dt;
         return;
     }
-    // device loop: solver.py:475-479
-    if (!(h.t < h.t_end && h.steps < h.max_steps)) { h.done = 1; return; }
-    const double rem = h.t_end - h.t;
-    if (rem < dt) dt = rem;
-    if (dt <= 1e-16) { h.done = 1; return; }
-    h.dt = dt;
-    h.decay = ch > 0.0 ? exp(-h.glm_alpha * ch * dt / h.min_dxdy) : 1.0;
-    h.pending = 1;
+    ctl_begin(h, dt);   // device loop: solver.py:475-479
 }
-
 
 __device__ __forceinline__ void step_ctl_block(StepCtl* ctl, const double* part_dt, int n_part, const double* part_bf,
                                                int n_bf, double dx, double dy, double host_dt, double cfl, int mode) {
@@ -829,92 +1112,6 @@ __global__ __launch_bounds__(256) void step_ctl_kernel(StepCtl* ctl, const doubl
                                                        const double* part_bf, int n_bf, double dx, double dy,
                                                        double host_dt, double cfl, int mode) {
     step_ctl_block(ctl, part_dt, n_part, part_bf, n_bf, dx, dy, host_dt, cfl, mode);
-}
-
-// Segment boundaries sized by cost.  A row whose x faces all took the
-// uniform-face path costs about 5/7 of one that solved them (a least-squares
-// fit of per-wavefront durations on the magnetic sieve, 18000 wavefronts at
-// steps 200, 800 and 1500: 1.9-2.0 us per uniform row against 2.66-2.74 us
-// per solved row, plus 5-8 us per wavefront), so row r of the wide strips
-// costs sum over strips of (5 or 7), and segment g (0 < g < n_segs) starts
-// after the row whose cost interval (P(r), P(r + 1)] holds the target
-// g total / n_segs.  With at least two rows per segment the targets lie
-// more than one row's cost apart (a row costs at most 7/5 of the mean row,
-// and 2 x 5 > 7), so the boundaries increase strictly and every segment
-// keeps at least one row.  One workgroup; larger grids than the LDS limits
-// below, or fewer than two rows per segment, keep the table as it is.
-constexpr int RB_MAX_ROWS = 8192;
-#ifndef AMHD_RB_W_UNIFORM
-#define AMHD_RB_W_UNIFORM 5
-#define AMHD_RB_W_SOLVED 7
-#endif
-constexpr int RB_W_UNIFORM = AMHD_RB_W_UNIFORM, RB_W_SOLVED = AMHD_RB_W_SOLVED;
-struct Rebalance {
-    const unsigned char* flags;   // [nx][RB_FLAG_PITCH]
-    int n_wide, nx, n_segs;
-    int* seg_i0;                  // [n_segs + 1]
-};
-
-__device__ __forceinline__ void rebalance_block(const Rebalance& rb) {
-    __shared__ int rcost[RB_MAX_ROWS];
-    __shared__ long wtot[4];
-    const int T = (int)blockDim.x, t = (int)threadIdx.x;
-    const int nx = rb.nx, ns = rb.n_wide, ng = rb.n_segs;
-    if (nx > RB_MAX_ROWS || nx < 2 * ng || ng < 2 || ns < 1 || ns > RB_FLAG_PITCH || T != 256) return;
-    // each thread's run of rows [r0, r1)
-    const int R = (nx + T - 1) / T;
-    const int r0 = t * R < nx ? t * R : nx, r1 = r0 + R < nx ? r0 + R : nx;
-    // row costs: the flag bytes (0 or 1, padding 0) of a row counted by
-    // popcount, 16 bytes per load, four rows' loads in flight at a time
-    const int words = (ns + 15) / 16;
-    long loc = 0;
-    for (int rb0 = r0; rb0 < r1; rb0 += 4) {
-        uint4 v[4][RB_FLAG_PITCH / 16];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-#pragma unroll
-            for (int w = 0; w < RB_FLAG_PITCH / 16; ++w) {
-                const bool ok = rb0 + u < r1 && w < words;
-                v[u][w] = ok ? *(const uint4*)(rb.flags + (long)(rb0 + u) * RB_FLAG_PITCH + 16 * w) : make_uint4(0, 0, 0, 0);
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            int uni = 0;
-#pragma unroll
-            for (int w = 0; w < RB_FLAG_PITCH / 16; ++w)
-                uni += __popc(v[u][w].x) + __popc(v[u][w].y) + __popc(v[u][w].z) + __popc(v[u][w].w);
-            if (rb0 + u < r1) {
-                const int c = uni * RB_W_UNIFORM + (ns - uni) * RB_W_SOLVED;
-                rcost[rb0 + u] = c;
-                loc += c;
-            }
-        }
-    }
-    // exclusive prefix of the runs: inclusive scan in each wavefront, then
-    // the wavefront totals
-    const int lane = t & 63, wv = t >> 6;
-    long inc = loc;
-    for (int d = 1; d < 64; d <<= 1) {
-        const long o = __shfl_up(inc, d, 64);
-        if (lane >= d) inc += o;
-    }
-    if (lane == 63) wtot[wv] = inc;
-    __syncthreads();
-    long P = inc - loc, total = 0;
-    for (int u = 0; u < 4; ++u) {
-        if (u < wv) P += wtot[u];
-        total += wtot[u];
-    }
-    for (int r = r0; r < r1; ++r) {
-        const long Pn = P + rcost[r];
-        for (long g = P * ng / total + 1; g <= Pn * ng / total && g < ng; ++g) rb.seg_i0[g] = r + 1;
-        P = Pn;
-    }
-    if (t == 0) {
-        rb.seg_i0[0] = 0;
-        rb.seg_i0[ng] = nx;
-    }
 }
 
 // Step control (workgroup 0) and, beside it, the segment boundaries of the
@@ -1200,7 +1397,7 @@ __global__ __launch_bounds__(WAVE* WPB) AMHD_STAGE_OCC void batch_stage_kernel(c
         }
         const BatchInst& d = insts[lo];
         const int cur = d.ctl->cur;
-        StageParams<real> a;
+        StageParams<real> a{};
         a.u0 = (const real*)d.buf[cur];
         a.us = STAGE == 1 ? (const real*)d.buf[cur] : STAGE == 2 ? (const real*)d.buf[(cur + 1) % 3]
                                                                 : (const real*)d.buf[(cur + 2) % 3];

@@ -71,7 +71,7 @@ void launch_stage_t(bool powell, bool glm, const StageParams<real>& a, dim3 grid
 template <class real, class P>
 hipError_t launch_stage_impl(int stage, bool powell, bool glm, const StageParams<real>& a, hipStream_t s) {
     const int n_waves = stage_wave_count(a);
-    const dim3 grid((n_waves + WPB - 1) / WPB), block(WAVE * WPB);
+    const dim3 grid((n_waves + WPB - 1) / WPB + (a.side ? 1 : 0)), block(WAVE * WPB);
     switch (stage) {
         case 0: launch_stage_t<real, P, 0>(powell, glm, a, grid, block, s); break;
         case 1: launch_stage_t<real, P, 1>(powell, glm, a, grid, block, s); break;

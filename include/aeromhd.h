@@ -119,15 +119,19 @@ int amhd_synchronize(void* ctx);
 /* Device-resident time loop of MHDSolver.run (solver.py:475-482) without
  * diagnostics: up to n_steps steps, stopping at t_end or max_steps (counted
  * in *step_inout) like the reference loop.  dt, ch and the psi decay are
- * computed on the device; no host round trip per step.  use_graph: replay
- * a captured hipGraph of the step sequence.  Synchronises at the end. */
+ * computed on the device; no host round trip per step.  A step is three
+ * launches: the stage kernels carry the step control (the first stage
+ * reduces the CFL maxima and begins the step, the third closes it).
+ * use_graph: replay a captured hipGraph of the step sequence.  Synchronises
+ * at the end. */
 int amhd_run(void* ctx, int n_steps, double cfl, double t_end, int max_steps, int use_graph, double* t_inout,
              int* step_inout, double* last_dt);
 
 /* amhd_run without graph replay and with a HIP event pair around every
  * launch on the context stream: stage_ms[s] receives the summed device time
- * of the stage-(s+1) kernel over the steps run, *ctl_ms that of the step
- * control kernel (may be NULL).  For measurement. */
+ * of the stage-(s+1) kernel over the steps run; *ctl_ms (may be NULL)
+ * receives 0, the device loop having no separate step-control launch.  For
+ * measurement. */
 int amhd_run_profiled(void* ctx, int n_steps, double cfl, double t_end, int max_steps, double* t_inout,
                       int* step_inout, double* last_dt, double stage_ms[3], double* ctl_ms);
 

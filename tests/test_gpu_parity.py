@@ -444,6 +444,60 @@ def test_graph_and_host_loops_agree(torch_gpu):
     assert out[0][1:] == out[1][1:]
 
 
+@pytest.mark.parametrize("precision,nx,ny", [("fp32", 256, 128), ("fp64", 128, 64)])
+def test_device_loop_equals_host_steps_to_t_end(torch_gpu, precision, nx, ny):
+    """The device loop, whose first and third stage kernels carry the step
+    control (CFL maxima reduced at the start of the first stage, the step
+    closed by the third), against host-driven steps with dt from the
+    step-control kernel (solver.py:475-482), over a run that ends at t_end
+    with a clipped last step; graph replay and eager launches alike.  FP32:
+    the same state, t, step count, last dt and boundary energy bit for bit.
+    FP64: the host-driven steps take the psi decay from the host's exp, the
+    device loop from the device's, which may differ in the last bit; the
+    step count is equal and everything else within 1e-13."""
+    from amhd_ics import cons_from_prim_host, magnetic_sieve
+    from solver import MHDSolver
+
+    cfg, W = magnetic_sieve(nx=nx, ny=ny, seed=5, precision=precision, max_steps=10 ** 6)
+    U0 = cons_from_prim_host(W, cfg.gamma)
+
+    def host_run(t_end, n_max):
+        s = MHDSolver(cfg)
+        s.U = U0
+        while s.t < t_end and s.step < n_max:
+            s.dt = min(s.compute_dt(), t_end - s.t)
+            if s.dt <= 1e-16:
+                break
+            s.step_ssprk3()
+            s.t += s.dt
+            s.step += 1
+        return s
+
+    probe = host_run(1e30, 17)
+    t16 = probe.t - probe.dt
+    t_end = t16 + 0.5 * probe.dt          # the 17th step is clipped to end here
+    h = host_run(t_end, 10 ** 6)
+    say(f"{precision} {nx}x{ny}: host steps {h.step}, t {h.t!r}, last dt {h.dt!r}")
+    assert h.step == 17 and h.t == t_end
+    ref = (h.ctx.download_numpy(), h.t, h.step, h.dt, h._cumulative_boundary_energy)
+    for graph in (True, False):
+        cfg.t_end = t_end
+        s = MHDSolver(cfg)
+        s.U = U0
+        ran = s.advance(40, use_graph=graph)
+        cfg.t_end = 1e30
+        say(f"device loop graph={graph}: {ran} steps, t {s.t!r}")
+        got = (s.ctx.download_numpy(), s.t, s.step, s.dt, s._cumulative_boundary_energy)
+        if precision == "fp32":
+            np.testing.assert_array_equal(got[0], ref[0])
+            assert got[1:] == ref[1:], (got[1:], ref[1:])
+        else:
+            assert rel_linf(got[0], ref[0]).max() <= 1e-13
+            assert got[2] == ref[2]
+            for g, r in zip(got[1:], ref[1:]):
+                assert abs(g - r) <= 1e-13 * max(abs(r), 1e-300), (got[1:], ref[1:])
+
+
 def test_transpose_symmetry_gpu(torch_gpu):
     """Step(T U) == T Step(U) bit for bit on a square periodic grid."""
     from config import Config
