@@ -504,6 +504,16 @@ __device__ __forceinline__ real hlld_face(const real (&wl)[NV], const real (&wr)
             if (GLM && (v == 4 || v == 8)) continue;   // set below
             f[v] = fk[v] + sks * (us[v] - uk[v]);
         }
+        if (star) {
+            // the mass and scalar components of F_K + S_K (U*_K - U_K) in
+            // fewer operations: rho*_K (S_K - SM) = rho_K (S_K - v_K) (the
+            // star density, physics.py:122-125) gives rho v + S (rho* - rho) =
+            // rho* SM, and C is the same in U_K and U*_K; the same flux up to
+            // rounding (and the density floor, which only a vacuum meets)
+            const real ms = rhos * sm;
+            f[0] = ms;
+            f[9] = ms * c;
+        }
         // S*_K < 0 (K = L) or >= 0 (K = R).  At S*_K = 0 the correction
         // below vanishes, so both sides may leave it out there: the test is
         // (K = L ? -S*_K : S*_K) > 0, and a face with Bx = 0 and SM = 0 (a
