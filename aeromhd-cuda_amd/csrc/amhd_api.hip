@@ -146,8 +146,9 @@ struct Ctx {
     const void* halo[2] = {nullptr, nullptr};   // x-slab halo buffers (lo, hi), caller-owned
     int* seg_i0 = nullptr;       // device [nx + 1]: first row of each segment, then nx
     int* seg_next = nullptr;     // device [nx + 1]: the table the device loop's stages 2 and 3 read
-    int balance = 0;             // size the segments by cost each step (FP32; amhd_set_balance)
-    unsigned char* rowflags = nullptr;   // device [nx][RB_FLAG_PITCH]: rows of uniform x faces
+    int* seg_first = nullptr;    // device [nx + 1]: the device loop's next first-stage table
+    int balance = 0;             // size the segments by cost each step (amhd_set_balance)
+    unsigned char* rowflags = nullptr;   // device [nx][RB_FLAG_PITCH]: rows equal to the next, per strip
 };
 
 
@@ -240,6 +241,8 @@ hipError_t upload_segments(Ctx* c) {
     hipError_t e = hipMemcpyAsync(c->seg_i0, i0.data(), sizeof(int) * i0.size(), hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess)
         e = hipMemcpyAsync(c->seg_next, i0.data(), sizeof(int) * i0.size(), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(c->seg_first, i0.data(), sizeof(int) * i0.size(), hipMemcpyHostToDevice, c->stream);
     return e == hipSuccess ? hipStreamSynchronize(c->stream) : e;
 }
 
@@ -295,11 +298,12 @@ hipError_t enqueue_step(Ctx* c, int cur, double dt, double ch, double decay, con
 // stage kernels (no control launch).  Stage 1 reduces the CFL partials of
 // the previous step (or of the state upload) in every workgroup and begins
 // the step from the committed block `ctl` into `ctl_next`; with cost-sized
-// segments its side workgroup re-sizes the next table (seg_next) from the
-// row flags.  Stages 2 and 3 read ctl_next and seg_next; the side workgroup
-// of stage 3 closes the step into `ctl` and copies seg_next to seg_i0 for
-// the next step's first stage.  No table or block is written while a
-// launch reads it.
+// segments its side workgroup sizes two tables from the row flags, with
+// the weights of each: seg_next for this step's stages 2 and 3 and
+// seg_first for the next step's stage 1.  Stages 2 and 3 read ctl_next and
+// seg_next; the side workgroup of stage 3 closes the step into `ctl` and
+// copies seg_first to seg_i0, the table stage 1 reads.  No table or block
+// is written while a launch reads it.
 // CFL partials the fused third stage leaves: one per strip workgroup.
 int loop_parts(const Ctx* c) { return (n_waves(c) + WPB - 1) / WPB; }
 
@@ -314,8 +318,12 @@ hipError_t launch_loop_stage(Ctx* c, int stage, const void* u0, const void* us, 
         a.part_in = c->part_dt;
         a.seg_i0 = c->seg_i0;
         if (bal) {
+            // this step's table for stages 2 and 3, the next step's for stage 1
             a.side = 1;
-            a.rb = Rebalance{c->rowflags, c->pk_groups ? c->n_strips - 1 : c->n_strips, c->nx, c->n_segs, c->seg_next};
+            const int nw = c->pk_groups ? c->n_strips - 1 : c->n_strips;
+            const bool f32 = sizeof(real) == 4;
+            a.rb = Rebalance{c->rowflags, nw, c->nx, c->n_segs, c->seg_next, f32 ? RB_W23 : RB_W23_F64};
+            a.rb1 = Rebalance{c->rowflags, nw, c->nx, c->n_segs, c->seg_first, f32 ? RB_W1 : RB_W1_F64};
         }
     } else {
         a.ctl = c->ctl_next;
@@ -326,6 +334,7 @@ hipError_t launch_loop_stage(Ctx* c, int stage, const void* u0, const void* us, 
             a.part_bf_in = c->p.bc_x_periodic ? nullptr : c->part_bf;
             a.n_part_next = loop_parts(c);
             a.seg_copy = bal ? c->seg_i0 : nullptr;
+            a.seg_copy_src = c->seg_first;
         }
     }
     if constexpr (sizeof(real) == 8) return launch_stage_f64(stage, c->p.powell != 0, true, a, c->stream);
@@ -348,7 +357,8 @@ hipError_t enqueue_ctl(Ctx* c, int n_part, double host_dt, double cfl, int mode,
     if (c->balance && mode != CTL_DT_ONLY) {
         // the step's last stage has recorded its row flags: size the next
         // step's segments (second workgroup, beside the step control)
-        Rebalance rb{c->rowflags, c->pk_groups ? c->n_strips - 1 : c->n_strips, c->nx, c->n_segs, c->seg_i0};
+        Rebalance rb{c->rowflags, c->pk_groups ? c->n_strips - 1 : c->n_strips, c->nx, c->n_segs, c->seg_i0,
+                     c->dtype == AMHD_F32 ? RB_W23 : RB_W23_F64};
         step_ctl_balance_kernel<<<2, 256, 0, c->stream>>>(ctl, c->part_dt, n_part, bf, c->n_strips, c->p.dx, c->p.dy,
                                                           host_dt, cfl, mode, rb);
     } else {
@@ -640,8 +650,9 @@ int amhd_create(void** out, int nx, int ny, int dtype, const amhd_params* p) {
     if (e == hipSuccess) e = hipMalloc(&c->ctl_next, sizeof(StepCtl));
     if (e == hipSuccess) e = hipMalloc(&c->seg_i0, sizeof(int) * (size_t)(nx + 1));
     if (e == hipSuccess) e = hipMalloc(&c->seg_next, sizeof(int) * (size_t)(nx + 1));
-    // cost-sized segments where uniform x faces make rows cheaper (FP32)
-    c->balance = dtype == AMHD_F32 ? 1 : 0;
+    if (e == hipSuccess) e = hipMalloc(&c->seg_first, sizeof(int) * (size_t)(nx + 1));
+    // cost-sized segments: rows that repeat the previous one are cheaper
+    c->balance = 1;
     if (e == hipSuccess) e = hipMalloc(&c->rowflags, (size_t)RB_FLAG_PITCH * nx);
     if (e == hipSuccess) e = hipMemsetAsync(c->rowflags, 0, (size_t)RB_FLAG_PITCH * nx, c->stream);
     if (e == hipSuccess) e = upload_segments(c);
@@ -675,6 +686,7 @@ int amhd_destroy(void* ctx) {
     if (c->ctl_next) hipFree(c->ctl_next);
     if (c->seg_i0) hipFree(c->seg_i0);
     if (c->seg_next) hipFree(c->seg_next);
+    if (c->seg_first) hipFree(c->seg_first);
     if (c->rowflags) hipFree(c->rowflags);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;

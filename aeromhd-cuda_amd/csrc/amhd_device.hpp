@@ -578,6 +578,23 @@ __device__ __forceinline__ bool all_zero(const real (&d)[NV]) {
     return (acc & 0x7fffffffu) == 0u;
 }
 
+// True when a and b hold the same bits in every component.  A computation
+// fed with the same bits returns the same bits, so a face whose inputs are
+// those of a face already solved may take that face's result.
+__device__ __forceinline__ bool same_bits(const float (&a)[NV], const float (&b)[NV]) {
+    unsigned acc = 0u;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) acc |= __float_as_uint(a[v]) ^ __float_as_uint(b[v]);
+    return acc == 0u;
+}
+__device__ __forceinline__ bool same_bits(const double (&a)[NV], const double (&b)[NV]) {
+    unsigned long long acc = 0ull;
+#pragma unroll
+    for (int v = 0; v < NV; ++v)
+        acc |= (unsigned long long)__double_as_longlong(a[v]) ^ (unsigned long long)__double_as_longlong(b[v]);
+    return acc == 0ull;
+}
+
 // y-face flux: rotate (vx,vy),(Bx,By), solve, rotate back (physics.py:373-379)
 template <class real, class P, bool GLM = false, bool FLOORED = false>
 __device__ __forceinline__ real hlld_face_y(const real (&wl)[NV], const real (&wr)[NV], real (&f)[NV],

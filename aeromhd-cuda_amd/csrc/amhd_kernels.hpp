@@ -108,12 +108,38 @@ __device__ __forceinline__ void ctl_begin(StepCtl& h, double dt) {
     h.pending = 1;
 }
 
-// Segment re-sizing (rebalance_block below): row flags in, boundary table out.
+// Segment re-sizing (rebalance_block below): row flags in, boundary table
+// out.  A row's cost is wb, plus wx where its x faces were solved, plus wy
+// where its y faces were (the previous row differs).
+struct RbWeights {
+    int wb, wx, wy;
+};
 struct Rebalance {
     const unsigned char* flags;   // [nx][RB_FLAG_PITCH]
     int n_wide, nx, n_segs;
     int* seg_i0;                  // [n_segs + 1]
+    RbWeights w;
 };
+// The weights of the first stage's table and of the table of stages 2 and
+// 3 (and of the host-driven step, whose three stages share one table).
+// FP32: stage 1 reads and writes two states per cell, so a row whose x and
+// y faces repeat the previous row's costs little; stages 2 and 3 move
+// three states per cell, and such a row costs about what its memory
+// traffic does.  FP64 is bound by its arithmetic in every stage.
+#ifndef AMHD_RB_W1
+#define AMHD_RB_W1 3, 2, 4
+#endif
+#ifndef AMHD_RB_W23
+#define AMHD_RB_W23 4, 2, 1
+#endif
+#ifndef AMHD_RB_W1_F64
+#define AMHD_RB_W1_F64 4, 3, 3
+#endif
+#ifndef AMHD_RB_W23_F64
+#define AMHD_RB_W23_F64 4, 3, 3
+#endif
+constexpr RbWeights RB_W1{AMHD_RB_W1}, RB_W23{AMHD_RB_W23};
+constexpr RbWeights RB_W1_F64{AMHD_RB_W1_F64}, RB_W23_F64{AMHD_RB_W23_F64};
 
 template <class real> struct StageParams {
     const real* u0;   // U^n (interior layout)
@@ -143,26 +169,28 @@ template <class real> struct StageParams {
     const real* halo_hi;
     int first_blocks;       // workgroups dispatched first onto each CU (the CU count); 0 = not known
     const int* seg_i0;      // [n_segs + 1] first row of each segment (null: seg*nx/n_segs)
-    // STAGE 3, wide strips: rowflags[i * RB_FLAG_PITCH + strip] = 1 where row
-    // i's x faces all took the uniform-face path in the strip's wavefront
-    // (null = none); the step control sizes the next step's segments from
-    // them
+    // STAGE 3, wide strips: rowflags[i * RB_FLAG_PITCH + strip] = 1 where
+    // rows i and i+1 hold the same bits in every lane of the strip's
+    // wavefront (FP32: where face i+1/2 took the uniform-face path there)
+    // (null = none); the next step's segments are sized from them
     unsigned char* rowflags;
     // Fused device loop (fuse = 1): STAGE 1 reduces the previous step's CFL
     // partials (part_in, ctl->n_part of them) in every workgroup, reads the
     // committed control block `ctl` and writes the step's block to ctl_out
     // (workgroup 0); STAGE 2 and 3 read the step's block through `ctl`.
     // side = 1: one workgroup after the strips does side work instead:
-    // STAGE 1 re-sizes the segments into rb.seg_i0 (a copy of seg_i0 where
-    // it cannot), STAGE 3 closes the step from `ctl` and part_bf_in into
-    // ctl_out (the committed block) and copies seg_i0 to seg_copy.
+    // STAGE 1 re-sizes the segments into rb.seg_i0 and, when rb1.seg_i0 is
+    // set, into rb1.seg_i0 with rb1's weights (a copy of seg_i0 where it
+    // cannot), STAGE 3 closes the step from `ctl` and part_bf_in into
+    // ctl_out (the committed block) and copies seg_copy_src to seg_copy.
     int fuse, side;
     const double* part_in;
     StepCtl* ctl_out;
     const double* part_bf_in;
     int n_part_next;   // STAGE 3 side: the CFL partials the stage writes (one per strip workgroup)
-    Rebalance rb;
+    Rebalance rb, rb1;
     int* seg_copy;
+    const int* seg_copy_src;
 };
 
 // Step values a fused stage kernel hands to its wavefronts.
@@ -283,37 +311,36 @@ __device__ __forceinline__ double bflux_term(const real (&u)[NV], double gamma, 
     return ((ud[EN] + pt) * w[iVX] - w[iBX] * vb) * dy;
 }
 
-// Segment boundaries sized by cost.  A row whose x faces all took the
-// uniform-face path costs about 5/7 of one that solved them (a least-squares
-// fit of per-wavefront durations on the magnetic sieve, 18000 wavefronts at
-// steps 200, 800 and 1500: 1.9-2.0 us per uniform row against 2.66-2.74 us
-// per solved row, plus 5-8 us per wavefront), so row r of the wide strips
-// costs sum over strips of (5 or 7), and segment g (0 < g < n_segs) starts
-// after the row whose cost interval (P(r), P(r + 1)] holds the target
-// g total / n_segs.  With at least two rows per segment the targets lie
-// more than one row's cost apart (a row costs at most 7/5 of the mean row,
-// and 2 x 5 > 7), so the boundaries increase strictly and every segment
-// keeps at least one row.  One workgroup; larger grids than the LDS limits
-// below, or fewer than two rows per segment, keep the table as it is (returns false).
+// Segment boundaries sized by cost.  Row r of a strip costs a base wb, plus
+// wx where its x faces were solved (flag 0: not every face of the
+// wavefront took the uniform-face path), plus wy where row r-1's were
+// (then rows r-1 and r differ somewhere and row r solves its y faces; a
+// flag 1 on row r-1 means the two rows hold the same bits).  Row r of the
+// wide strips costs the sum over strips, and segment g (0 < g < n_segs)
+// starts after the row whose cost interval (P(r), P(r + 1)] holds the
+// target g total / n_segs.  When the cheapest row times the rows per segment
+// exceeds the dearest row (nx wb > n_segs (wb + wx + wy)), the
+// targets lie more than one row's cost apart, so the boundaries increase
+// strictly and every segment keeps at least one row.  One workgroup; larger
+// grids than the LDS limits below, or fewer rows per segment, keep the
+// table as it is (returns false).
 constexpr int RB_MAX_ROWS = 8192;
-#ifndef AMHD_RB_W_UNIFORM
-#define AMHD_RB_W_UNIFORM 5
-#define AMHD_RB_W_SOLVED 7
-#endif
-constexpr int RB_W_UNIFORM = AMHD_RB_W_UNIFORM, RB_W_SOLVED = AMHD_RB_W_SOLVED;
 __device__ __forceinline__ bool rebalance_block(const Rebalance& rb) {
     __shared__ int rcost[RB_MAX_ROWS];
     __shared__ long wtot[4];
     const int T = (int)blockDim.x, t = (int)threadIdx.x;
     const int nx = rb.nx, ns = rb.n_wide, ng = rb.n_segs;
-    if (nx > RB_MAX_ROWS || nx < 2 * ng || ng < 2 || ns < 1 || ns > RB_FLAG_PITCH || T != 256) return false;
+    const int wb = rb.w.wb, wx = rb.w.wx, wy = rb.w.wy;
+    if (nx > RB_MAX_ROWS || wb < 1 || wx < 0 || wy < 0 || (long)nx * wb <= (long)ng * (wb + wx + wy) || ng < 2 ||
+        ns < 1 || ns > RB_FLAG_PITCH || T != 256)
+        return false;
     // each thread's run of rows [r0, r1)
     const int R = (nx + T - 1) / T;
     const int r0 = t * R < nx ? t * R : nx, r1 = r0 + R < nx ? r0 + R : nx;
-    // row costs: the flag bytes (0 or 1, padding 0) of a row counted by
-    // popcount, 16 bytes per load, four rows' loads in flight at a time
+    // uniform strips per row: the flag bytes (0 or 1, padding 0) of a row
+    // counted by popcount, 16 bytes per load, four rows' loads in flight at
+    // a time
     const int words = (ns + 15) / 16;
-    long loc = 0;
     for (int rb0 = r0; rb0 < r1; rb0 += 4) {
         uint4 v[4][RB_FLAG_PITCH / 16];
 #pragma unroll
@@ -330,12 +357,21 @@ __device__ __forceinline__ bool rebalance_block(const Rebalance& rb) {
 #pragma unroll
             for (int w = 0; w < RB_FLAG_PITCH / 16; ++w)
                 uni += __popc(v[u][w].x) + __popc(v[u][w].y) + __popc(v[u][w].z) + __popc(v[u][w].w);
-            if (rb0 + u < r1) {
-                const int c = uni * RB_W_UNIFORM + (ns - uni) * RB_W_SOLVED;
-                rcost[rb0 + u] = c;
-                loc += c;
-            }
+            if (rb0 + u < r1) rcost[rb0 + u] = uni;
         }
+    }
+    // row costs in place, each from its own count and the previous row's
+    // (read before any thread overwrites it; no row before row 0)
+    __syncthreads();
+    int prev = r0 > 0 && r0 < r1 ? rcost[r0 - 1] : 0;
+    __syncthreads();
+    long loc = 0;
+    for (int r = r0; r < r1; ++r) {
+        const int uni = rcost[r];
+        const int c = ns * wb + (ns - uni) * wx + (ns - prev) * wy;
+        prev = uni;
+        rcost[r] = c;
+        loc += c;
     }
     // exclusive prefix of the runs: inclusive scan in each wavefront, then
     // the wavefront totals
@@ -441,6 +477,12 @@ __device__ __forceinline__ void stage_side(const StageParams<real>& a) {
         if (!rebalance_block(a.rb)) {
             for (int k = threadIdx.x; k <= a.n_segs; k += blockDim.x) a.rb.seg_i0[k] = a.seg_i0[k];
         }
+        if (a.rb1.seg_i0) {
+            __syncthreads();   // the second table reuses the first one's shared arrays
+            if (!rebalance_block(a.rb1)) {
+                for (int k = threadIdx.x; k <= a.n_segs; k += blockDim.x) a.rb1.seg_i0[k] = a.seg_i0[k];
+            }
+        }
     } else if constexpr (STAGE == 3) {
         // every stage of the step has read the step's block: close the step
         if (threadIdx.x < 64) {
@@ -455,7 +497,7 @@ __device__ __forceinline__ void stage_side(const StageParams<real>& a) {
             }
         }
         if (a.seg_copy) {
-            for (int k = threadIdx.x; k <= a.n_segs; k += blockDim.x) a.seg_copy[k] = a.seg_i0[k];
+            for (int k = threadIdx.x; k <= a.n_segs; k += blockDim.x) a.seg_copy[k] = a.seg_copy_src[k];
         }
     }
 }
@@ -593,10 +635,10 @@ __device__ __forceinline__ void stage_wave(const StageParams<real>& a, const int
     real A[NV], B[NV], C[NV];
     real QA[NV], QB[NV], QC[NV];
     real F0[NV], F1[NV], F2[NV];
-    real xl[NV], dxp[NV];   // left state of face i+1/2; W_{i+1} - W_i
-    real bxm;               // Bx of row i-1 (Powell)
+    real xl[NV];   // left state of face i+1/2
+    real bxm;      // Bx of row i-1 (Powell)
     {
-        real wm2[NV], wm1[NV], fa[NV], fb[NV], dm0[NV];
+        real wm2[NV], wm1[NV], fa[NV], fb[NV];
         real um2[NV], um1[NV];
         // fused first stage: the CFL partials (cache-resident) are loaded
         // first and reduced while the segment's rows, loaded next, are on
@@ -616,16 +658,11 @@ __device__ __forceinline__ void stage_wave(const StageParams<real>& a, const int
         cons_to_prim<real, P>(um1, wm1, kc.gamma, psi_ghost(i0 - 1));
         cons_to_prim<real, P>(QA, A, kc.gamma, psi_ghost(i0));
         cons_to_prim<real, P>(QB, B, kc.gamma, psi_ghost(i0 + 1));
-#pragma unroll
-        for (int v = 0; v < NV; ++v) {
-            dm0[v] = A[v] - wm1[v];
-            dxp[v] = B[v] - A[v];
-        }
         // the uniform-face rule of the row loop, for face i0-1/2: a lane whose
         // face is uniform gets F(W_i0) and the left state W_i0 of the next
         // face, whichever branch its wavefront takes
         bool uni0 = false;
-        if constexpr (XSHORT) uni0 = all_zero(dm0);
+        if constexpr (XSHORT) uni0 = same_bits(A, wm1);
         if (XSHORT && __builtin_amdgcn_ballot_w64(!uni0) == 0) {
             phys_flux_x(A, F0, kc);
 #pragma unroll
@@ -634,8 +671,9 @@ __device__ __forceinline__ void stage_wave(const StageParams<real>& a, const int
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
                 const real dmm = wm1[v] - wm2[v];
-                const real h_m1 = vl_half_d<real, P>(dmm, dm0[v]);
-                const real h_0 = vl_half_d<real, P>(dm0[v], dxp[v]);
+                const real dm0 = A[v] - wm1[v];
+                const real h_m1 = vl_half_d<real, P>(dmm, dm0);
+                const real h_0 = vl_half_d<real, P>(dm0, B[v] - A[v]);
                 fa[v] = wm1[v] + h_m1;
                 fb[v] = A[v] - h_0;
                 xl[v] = A[v] + h_0;
@@ -659,6 +697,26 @@ __device__ __forceinline__ void stage_wave(const StageParams<real>& a, const int
     real upre[NV];
     load_in(i0 + 2, upre);
 
+    // Rows that repeat.  Where rows i-1 and i hold the same bits in every
+    // lane of the wavefront (y_same, wave-uniform), the y faces of row i
+    // have the inputs of those of row i-1 and so their fluxes: fy, by_prev
+    // and by_next keep row i-1's.  A flow that depends on y alone keeps its
+    // rows identical bit for bit (each gets the same update), so this skips
+    // the y solves of every row but the first of such a run.
+    real fy[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) fy[v] = real(0);
+    real by_prev = real(0), by_next = real(0);
+    bool y_same = false;
+    // Without the uniform-face path (FP64), the x faces repeat too: where
+    // rows i-2 .. i+2 hold the same bits in every lane, face i+1/2 has the
+    // inputs of face i-1/2 (the same states and slopes), and the left state
+    // of face i+3/2 is that of face i+1/2.  idh bit k: rows i-k and i-k+1
+    // are the same (row i = the row about to be computed); none is known
+    // before the segment's first row but rows i0, i0+1.
+    unsigned idh = 0u;
+    if constexpr (!XSHORT) idh = __builtin_amdgcn_ballot_w64(!same_bits(A, B)) == 0 ? 1u : 0u;
+
     // One output row i.  w0 = W_i, wp1 = W_{i+1}; wp2 receives W_{i+2};
     // ucur = U_i (stage input); qp2 receives U_{i+2}; fxl = F_{i-1/2}; fxr
     // receives F_{i+1/2}.
@@ -674,31 +732,47 @@ __device__ __forceinline__ void stage_wave(const StageParams<real>& a, const int
         real ubase[NV];
         if constexpr (STAGE >= 2) load_row(r_u0, i, ubase);
 
+        // FP64: which faces of row i repeat those of row i-1
+        bool xrep = false;
+        unsigned row_flag = 0u;
+        if constexpr (!XSHORT) {
+            const bool id1 = __builtin_amdgcn_ballot_w64(!same_bits(wp1, wp2)) == 0;   // rows i+1, i+2
+            row_flag = idh & 1u;             // rows i, i+1
+            y_same = (idh & 2u) != 0u;       // rows i-1, i
+            xrep = id1 && (idh & 7u) == 7u;  // rows i-2 .. i+2
+            idh = ((idh << 1) | (id1 ? 1u : 0u)) & 7u;
+        }
+
         // ---- x face i+1/2: slope at row i+1 from the differences
-        // Uniform faces (FP32): where W_i == W_{i+1} in all ten components
-        // (dxp == 0), both van Leer slopes next to the face are 0, so its two
-        // states equal that cell and its flux is the physical flux F(W_{i+1});
-        // the left state of face i+3/2 is W_{i+1}.  Every lane whose face is
-        // uniform gets exactly that, whichever branch its wavefront takes; a
-        // wavefront whose faces are all uniform skips the slopes and the
-        // HLLD solve.
+        // Uniform faces (FP32): where W_i and W_{i+1} hold the same bits in
+        // all ten components, both van Leer slopes next to the face are 0, so
+        // its two states equal that cell and its flux is the physical flux
+        // F(W_{i+1}); the left state of face i+3/2 is W_{i+1}.  Every lane
+        // whose face is uniform gets exactly that, whichever branch its
+        // wavefront takes; a wavefront whose faces are all uniform skips the
+        // slopes and the HLLD solve.  If face i-1/2 was uniform in every lane
+        // too (y_same), F(W_{i+1}) is F(W_i), the flux of that face.
         bool uni = false;
-        if constexpr (XSHORT) uni = all_zero(dxp);
+        if constexpr (XSHORT) uni = same_bits(w0, wp1);
         const bool all_uni = XSHORT && __builtin_amdgcn_ballot_w64(!uni) == 0;
         if (all_uni) {
 #pragma unroll
-            for (int v = 0; v < NV; ++v) {
-                dxp[v] = wp2[v] - wp1[v];
-                xl[v] = wp1[v];
+            for (int v = 0; v < NV; ++v) xl[v] = wp1[v];
+            if (y_same) {
+#pragma unroll
+                for (int v = 0; v < NV; ++v) fxr[v] = fxl[v];
+            } else {
+                phys_flux_x(wp1, fxr, kc);
             }
-            phys_flux_x(wp1, fxr, kc);
+        } else if (xrep) {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) fxr[v] = fxl[v];
         } else {
             real xr[NV], xn[NV];
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
                 const real dn = wp2[v] - wp1[v];
-                const real h1 = vl_half_d<real, P>(dxp[v], dn);
-                dxp[v] = dn;
+                const real h1 = vl_half_d<real, P>(wp1[v] - w0[v], dn);
                 xr[v] = wp1[v] - h1;
                 xn[v] = wp1[v] + h1;
             }
@@ -716,14 +790,15 @@ __device__ __forceinline__ void stage_wave(const StageParams<real>& a, const int
             }
         }
 
-        if constexpr (STAGE == 3 && XSHORT && !PK) {
-            if (a.rowflags && lane == 0) a.rowflags[(long)i * RB_FLAG_PITCH + strip] = all_uni ? 1 : 0;
+        if constexpr (STAGE == 3 && !PK) {
+            // rows i and i+1 the same: the uniform-face path in every lane (FP32)
+            if (a.rowflags && lane == 0)
+                a.rowflags[(long)i * RB_FLAG_PITCH + strip] = XSHORT ? (all_uni ? 1 : 0) : (unsigned char)row_flag;
         }
 
-        // ---- y face j+1/2 of row i (neighbours through DPP lane shifts)
-        real fy[NV];
-        real by_prev, by_next;
-        {
+        // ---- y face j+1/2 of row i (neighbours through DPP lane shifts),
+        // unless row i repeats row i-1
+        if (!y_same) {
             real yl[NV], yr[NV];
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
@@ -738,6 +813,8 @@ __device__ __forceinline__ void stage_wave(const StageParams<real>& a, const int
             floor_face<real, P>(yl); floor_face<real, P>(yr);
             hlld_face_y<real, P, GLM, true>(yl, yr, fy, kc);
         }
+        // does row i+1 repeat row i?
+        if constexpr (XSHORT) y_same = all_uni;
 
         // ---- RHS: ((0 - dFx/dx) - dFy/dy) + S   (solver.py:297-306)
         real divb = real(0);

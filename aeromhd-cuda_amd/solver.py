@@ -94,8 +94,8 @@ class Context:
         _L.check(self.lib.amhd_set_packing(self.h, int(max_groups)), "amhd_set_packing")
 
     def set_balance(self, on: bool):
-        """Size the segments by cost each step (default on for FP32
-        contexts); results do not depend on it."""
+        """Size the segments by cost each step (default on); results do
+        not depend on it."""
         _L.check(self.lib.amhd_set_balance(self.h, 1 if on else 0), "amhd_set_balance")
 
     def upload(self, U) -> None:

@@ -4,13 +4,15 @@
 Workload (default): the magnetic-sieve RMI on a 2048 x 1024 grid in FP32 --
 MUSCL/van Leer + HLLD + GLM psi transport + Powell source + SSP-RK3 + CFL
 reduction, dt computed on the device every step.  A "step" is one full
-SSP-RK3 step of the whole grid (three fused stage launches plus the step
-control launch).  Inputs are resident in HBM before the timed region.
+SSP-RK3 step of the whole grid (three fused stage launches; the first and
+the third carry the step control).  Inputs are resident in HBM before the
+timed region.
 
 Prints ONE JSON line: Mcell-updates/s, the roofline of the fused stage
 kernel (algorithmic bytes 8 * 10 * sizeof(real) per cell-update divided by
 the stage kernels' device time, HIP events on the context stream over the
-timed region), and the CPU oracle timed on this host on a bounded sample.
+timed region, replayed from the state the timed steps start from), and the
+CPU oracle timed on this host on a bounded sample.
 Secondary records: the 1024x512 RMI in FP64 and FP32, the batched form (64
 instances), the x-slab decomposition (4 slabs with halo exchange, one
 GPU), and the learned closure's fine-equivalent rate over the FP64 fine
@@ -76,6 +78,9 @@ def gpu_measure(name: str, steps: int, warmup: int, seed: int, rows: int | None,
 
         amhd_tune.tune_single(s, use_cache=False)   # times the candidates on a copy of this state
     s.advance(warmup)
+    # the state the timed steps start from (FP32 values round-trip exactly
+    # through the float64 copy), so that the roofline pass times the same steps
+    snap, t_snap, step_snap = s.ctx.download(), s.t, s.step
     # timed region 1 (value): the product loop -- device dt, hipGraph replay
     torch.cuda.synchronize()
     g0 = time.perf_counter()
@@ -83,8 +88,13 @@ def gpu_measure(name: str, steps: int, warmup: int, seed: int, rows: int | None,
     torch.cuda.synchronize()
     gwall = time.perf_counter() - g0
     assert n == steps, "device loop stopped early"
-    # timed region 2 (roofline): same loop launched without the graph, with a
-    # HIP event pair around every launch on the context stream
+    t_graph = s.t
+    # timed region 2 (roofline): the same steps again from the same state,
+    # launched without the graph, with a HIP event pair around every launch
+    # on the context stream
+    s.ctx.upload(snap)
+    s.t, s.step = t_snap, step_snap
+    del snap
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     t, step, last, stage_ms, _ = s.ctx.run_profiled(steps, cfg.cfl, cfg.t_end, cfg.max_steps, s.t, s.step)
@@ -105,6 +115,7 @@ def gpu_measure(name: str, steps: int, warmup: int, seed: int, rows: int | None,
         "achieved_gbs": bytes_step / kern_s / 1e9, "bytes_per_step": bytes_step,
         "bytes_per_launch": [2 * NVAR * esize * cells, 3 * NVAR * esize * cells, 3 * NVAR * esize * cells],
         "finite": all(x == x for x in sums), "t": t, "dt": last,
+        "same_steps": t == t_graph,   # the roofline pass ended where the timed graph run did
     }
 
 
@@ -355,7 +366,7 @@ def main():
     secondary = []
     if not args.no_secondary and args.workload == "sieve2048":
         # the same workload late in the run: the mixing layer has developed
-        # and fewer rows are uniform in x
+        # and fewer rows repeat the row beside them
         log("[bench] secondary sieve2048 late (after 1500 untimed steps)")
         x = gpu_measure("sieve2048", args.steps, 1500, args.seed, args.rows or None, args.packing)
         secondary.append({"workload": x["desc"] + ", timed after 1500 untimed steps", "value": round(x["mcups"], 2),
@@ -423,6 +434,7 @@ def main():
                      "algorithmic_bytes_per_launch": [int(b) for b in m["bytes_per_launch"]],
                      "avg_launch_bytes": int(avg_launch_bytes),
                      "stage_avg_us": [round(u, 2) for u in m["stage_avg_us"]],
+                     "same_steps_as_value": m["same_steps"],
                      "copy_GBps": round(copy_gbs, 1),
                      "frac_of_copy": round(m["achieved_gbs"] / copy_gbs, 4),
                      "copy_GBps_80MiB": round(copy_ws_gbs, 1)},

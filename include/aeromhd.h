@@ -68,11 +68,12 @@ int amhd_get_rows(void* ctx);
 int amhd_set_packing(void* ctx, int max_groups);
 
 /* Cost-sized segments (no counterpart in the reference): each step the
- * last stage records which rows' x faces were all uniform (cheap: the
- * physical flux instead of the HLLD solve), and the step control moves the
- * segment boundaries so that every wavefront's segment costs about the
- * same.  Default 1 for FP32 contexts (where uniform faces are cheaper), 0
- * for FP64.  Results do not depend on it. */
+ * last stage records, per strip, which rows hold the same bits as the next
+ * row (such a row takes its y-face fluxes from the row before, and its x
+ * faces are cheap: the physical flux in FP32, the previous face's flux in
+ * FP64 once five rows repeat), and the segment boundaries move so that
+ * every wavefront's segment costs about the same.  Default 1.  Results do
+ * not depend on it. */
 int amhd_set_balance(void* ctx, int on);
 
 /* MHDSolver.U assignment / read (solver.py:83, 139).  U: ghosted float64
