@@ -248,20 +248,22 @@ def test_packed_last_strip_is_bitwise_neutral(torch_gpu, precision, nx, ny, grou
     assert out[0][1] == out[1][1] and out[0][2] == out[1][2]
 
 
-@pytest.mark.parametrize("nx,ny,rows", [(2048, 1024, 0), (400, 131, 12), (256, 64, 5)])
-def test_cost_sized_segments_are_bitwise_neutral(torch_gpu, nx, ny, rows):
-    """FP32 segments re-sized every step from the rows' costs (uniform x
-    faces cost less) against fixed segments: 30 steps of the magnetic sieve
-    end with the same state, time and boundary energy bit for bit, through
-    the device loop and through the host-driven step loop."""
+@pytest.mark.parametrize("precision,nx,ny,rows", [("fp32", 2048, 1024, 0), ("fp32", 400, 131, 12),
+                                                  ("fp32", 256, 64, 5), ("fp64", 1024, 512, 0),
+                                                  ("fp64", 400, 131, 12)])
+def test_cost_sized_segments_are_bitwise_neutral(torch_gpu, precision, nx, ny, rows):
+    """Segments re-sized every step from the rows' costs (rows that repeat
+    their neighbour cost less) against fixed segments: 30 steps of the
+    magnetic sieve end with the same state, time and boundary energy bit for
+    bit, through the device loop and through the host-driven step loop."""
     from amhd_ics import cons_from_prim_host, magnetic_sieve
     from solver import MHDSolver
 
-    cfg, W = magnetic_sieve(nx=nx, ny=ny, seed=11, precision="fp32", max_steps=10 ** 6)
+    cfg, W = magnetic_sieve(nx=nx, ny=ny, seed=11, precision=precision, max_steps=10 ** 6)
     U0 = cons_from_prim_host(W, cfg.gamma)
     out = []
     for bal, host in ((True, False), (False, False), (True, True), (False, True)):
-        say(f"cost-sized segments {bal} host loop {host}: {nx}x{ny} rows {rows}")
+        say(f"cost-sized segments {bal} host loop {host}: {precision} {nx}x{ny} rows {rows}")
         s = MHDSolver(cfg)
         s.ctx.set_rows(rows)
         s.ctx.set_balance(bal)
@@ -278,6 +280,100 @@ def test_cost_sized_segments_are_bitwise_neutral(torch_gpu, nx, ny, rows):
     for a, b in ((out[0], out[1]), (out[2], out[3])):
         np.testing.assert_array_equal(a[0], b[0])
         assert a[1] == b[1] and a[2] == b[2]
+
+
+def repeated_rows_state(nx, ny, seed):
+    """Ghosted primitive state (NV, nx+4, ny+4) made of runs of 1-8 equal
+    rows (x = const), each run a smooth periodic profile in y.  Some runs equal the previous one in value but
+    not in bits (v_z = -0 against +0), and some differ from it in one cell
+    by one ulp of rho, so that consecutive rows are the same everywhere but
+    in one lane of a wavefront."""
+    rng = np.random.default_rng(seed)
+    y = (np.arange(ny + 4) - 2 + 0.5) / ny
+    W = np.empty((NV, nx + 4, ny + 4))
+    lengths = (3, 1, 2, 5, 1, 1, 4, 7, 2, 6, 1, 3, 8, 2, 1, 5)
+    prof = None
+    i = k = 0
+    while i < nx + 4:
+        n = min(lengths[k % len(lengths)], nx + 4 - i)
+        kind = "new" if prof is None else ("new", "zero_sign", "one_cell")[k % 3]
+        if kind == "new":
+            ph = rng.uniform(0, 2 * np.pi, size=NV)
+            prof = np.stack([
+                1.0 + 0.3 * np.sin(2 * np.pi * y + ph[0]),
+                0.2 * np.cos(2 * np.pi * y + ph[1]),
+                0.1 * np.sin(4 * np.pi * y + ph[2]),
+                np.zeros(ny + 4),
+                0.5 + 0.1 * np.sin(2 * np.pi * y + ph[4]),
+                0.3 * np.cos(2 * np.pi * y + ph[5]),
+                0.05 * np.sin(2 * np.pi * y + ph[6]),
+                1.0 + 0.2 * np.cos(2 * np.pi * y + ph[7]),
+                0.01 * np.sin(2 * np.pi * y + ph[8]),
+                0.5 + 0.4 * np.sin(2 * np.pi * y + ph[9]),
+            ])
+        elif kind == "zero_sign":
+            prof = prof.copy()
+            prof[3] = -prof[3]   # +0 <-> -0: equal values, different bits
+        else:
+            prof = prof.copy()
+            j = int(rng.integers(2, ny + 2))
+            prof[0, j] = np.nextafter(prof[0, j], 2.0)
+        W[:, i:i + n, :] = prof[:, None, :]
+        i += n
+        k += 1
+    return W
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_repeated_rows_at_any_tiling(torch_gpu, orc, precision):
+    """A row that holds the bits of the row before it takes that row's y-face
+    fluxes (and x faces where its neighbours repeat too).  On runs of 1-8
+    equal rows, with runs equal to the previous one in value only (-0 / +0)
+    or in all but one cell: one step at six tilings, balanced and fixed,
+    agrees bit for bit between tilings; FP64 equals the oracle's step and
+    FP32 is within 1e-5 of it (rel L-inf per variable).  Then 12 device-loop
+    steps from the same state agree bit for bit between tilings, FP64 within
+    1e-12 of the oracle's loop."""
+    from config import Config
+    from solver import MHDSolver
+
+    nx, ny = 151, 118
+    cfg = Config(nx=nx, ny=ny, x_min=0, x_max=1, y_min=0, y_max=ny / nx, gamma=5 / 3, cfl=0.3,
+                 bc_x_type="extrapolation", bc_y_type="periodic", B_transverse=0.25, powell_source=True,
+                 precision=precision, t_end=1e9, max_steps=10 ** 6)
+    U = prim_to_cons_np(repeated_rows_state(nx, ny, seed=5), cfg.gamma)
+    if precision == "fp32":
+        U = U.astype(np.float32).astype(np.float64)
+    dt, ch = orc.compute_dt(orc.cons_to_prim(U, cfg.gamma), cfg)
+    Uo, _ = orc.step(U, cfg, dt, ch)
+    one, loop = [], []
+    for rows, bal in ((0, True), (0, False), (1, True), (3, False), (5, True), (16, True)):
+        say(f"repeated rows {precision}: rows {rows} balance {bal}")
+        s = MHDSolver(cfg)
+        s.ctx.set_rows(rows)
+        s.ctx.set_balance(bal)
+        s.U = U
+        s.dt, cfg.glm_ch = dt, ch
+        s.step_ssprk3()
+        one.append(s.ctx.download_numpy())
+        s.U = U
+        s.t, s.step = 0.0, 0
+        assert s.advance(12) == 12
+        loop.append((s.ctx.download_numpy(), s.t))
+    for a in one[1:]:
+        np.testing.assert_array_equal(a.view(np.uint64), one[0].view(np.uint64))
+    for a, t in loop[1:]:
+        np.testing.assert_array_equal(a.view(np.uint64), loop[0][0].view(np.uint64))
+        assert t == loop[0][1]
+    err = rel_linf(one[0], Uo)
+    say(f"repeated rows {precision}: one step rel L-inf {err.max():.3e}")
+    if precision == "fp64":
+        np.testing.assert_array_equal(one[0], Uo)
+        Ul, tl, sl, _, _ = orc.run_loop(U, cfg, 12)
+        assert sl == 12
+        assert rel_linf(loop[0][0], Ul).max() <= TOL64
+    else:
+        assert err.max() <= 1e-5, err
 
 
 def test_fp32_uniform_state_stays_uniform(torch_gpu):
