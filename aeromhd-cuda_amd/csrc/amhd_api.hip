@@ -308,7 +308,8 @@ hipError_t enqueue_step(Ctx* c, int cur, double dt, double ch, double decay, con
 int loop_parts(const Ctx* c) { return (n_waves(c) + WPB - 1) / WPB; }
 
 template <class real>
-hipError_t launch_loop_stage(Ctx* c, int stage, const void* u0, const void* us, void* uo) {
+hipError_t launch_loop_stage(Ctx* c, int stage, const void* u0, const void* us, void* uo, hipEvent_t ev0 = nullptr,
+                             hipEvent_t ev1 = nullptr) {
     StageParams<real> a = stage_args<real>(c, stage, (const real*)u0, (const real*)us, (real*)uo);
     a.fuse = 1;
     const bool bal = c->balance != 0;
@@ -337,8 +338,8 @@ hipError_t launch_loop_stage(Ctx* c, int stage, const void* u0, const void* us, 
             a.seg_copy_src = c->seg_first;
         }
     }
-    if constexpr (sizeof(real) == 8) return launch_stage_f64(stage, c->p.powell != 0, true, a, c->stream);
-    else return launch_stage_f32(stage, c->p.powell != 0, true, a, c->stream);
+    if constexpr (sizeof(real) == 8) return launch_stage_f64(stage, c->p.powell != 0, true, a, c->stream, ev0, ev1);
+    else return launch_stage_f32(stage, c->p.powell != 0, true, a, c->stream, ev0, ev1);
 }
 
 hipError_t enqueue_loop_step(Ctx* c, int cur) {
@@ -956,8 +957,8 @@ int amhd_run_profiled(void* ctx, int n_steps, double cfl, double t_end, int max_
     h.n_part = c->n_part;
     HIP_TRY(hipMemcpyAsync(c->ctl, &h, sizeof h, hipMemcpyHostToDevice, c->stream));
     const int nw = loop_parts(c);
-    // two events around each of the 3 launches of a step (the stage kernels
-    // carry the step control)
+    // the start and end of each of the 3 launches of a step (the stage
+    // kernels carry the step control), recorded by the dispatches themselves
     std::vector<hipEvent_t> ev((size_t)n_steps * 6);
     for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
     hipError_t err = hipSuccess;
@@ -968,18 +969,15 @@ int amhd_run_profiled(void* ctx, int n_steps, double cfl, double t_end, int max_
         void* u2 = c->buf[(cur + 2) % 3];
         hipEvent_t* e = &ev[(size_t)st * 6];
         const bool f64 = c->dtype == AMHD_F64;
-        if (err == hipSuccess) err = hipEventRecord(e[0], c->stream);
         if (err == hipSuccess)
-            err = f64 ? launch_loop_stage<double>(c, 1, u0, u0, u1) : launch_loop_stage<float>(c, 1, u0, u0, u1);
-        if (err == hipSuccess) err = hipEventRecord(e[1], c->stream);
-        if (err == hipSuccess) err = hipEventRecord(e[2], c->stream);
+            err = f64 ? launch_loop_stage<double>(c, 1, u0, u0, u1, e[0], e[1])
+                      : launch_loop_stage<float>(c, 1, u0, u0, u1, e[0], e[1]);
         if (err == hipSuccess)
-            err = f64 ? launch_loop_stage<double>(c, 2, u0, u1, u2) : launch_loop_stage<float>(c, 2, u0, u1, u2);
-        if (err == hipSuccess) err = hipEventRecord(e[3], c->stream);
-        if (err == hipSuccess) err = hipEventRecord(e[4], c->stream);
+            err = f64 ? launch_loop_stage<double>(c, 2, u0, u1, u2, e[2], e[3])
+                      : launch_loop_stage<float>(c, 2, u0, u1, u2, e[2], e[3]);
         if (err == hipSuccess)
-            err = f64 ? launch_loop_stage<double>(c, 3, u0, u2, u1) : launch_loop_stage<float>(c, 3, u0, u2, u1);
-        if (err == hipSuccess) err = hipEventRecord(e[5], c->stream);
+            err = f64 ? launch_loop_stage<double>(c, 3, u0, u2, u1, e[4], e[5])
+                      : launch_loop_stage<float>(c, 3, u0, u2, u1, e[4], e[5]);
     }
     if (err == hipSuccess) err = hipMemcpyAsync(&h, c->ctl, sizeof h, hipMemcpyDeviceToHost, c->stream);
     if (err == hipSuccess) err = hipStreamSynchronize(c->stream);

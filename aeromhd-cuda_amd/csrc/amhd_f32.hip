@@ -8,8 +8,9 @@
 
 namespace amhd {
 
-hipError_t launch_stage_f32(int stage, bool powell, bool glm, const StageParams<float>& a, hipStream_t s) {
-    return launch_stage_impl<float, FastMath<float>>(stage, powell, glm, a, s);
+hipError_t launch_stage_f32(int stage, bool powell, bool glm, const StageParams<float>& a, hipStream_t s,
+                            hipEvent_t ev0, hipEvent_t ev1) {
+    return launch_stage_impl<float, FastMath<float>>(stage, powell, glm, a, s, ev0, ev1);
 }
 
 hipError_t launch_batch_stage_f32(int stage, const BatchInst* insts, int n_inst, int n_tasks, int resident,

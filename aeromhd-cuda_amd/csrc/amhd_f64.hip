@@ -7,8 +7,9 @@
 
 namespace amhd {
 
-hipError_t launch_stage_f64(int stage, bool powell, bool glm, const StageParams<double>& a, hipStream_t s) {
-    return launch_stage_impl<double, ExactMath<double>>(stage, powell, glm, a, s);
+hipError_t launch_stage_f64(int stage, bool powell, bool glm, const StageParams<double>& a, hipStream_t s,
+                            hipEvent_t ev0, hipEvent_t ev1) {
+    return launch_stage_impl<double, ExactMath<double>>(stage, powell, glm, a, s, ev0, ev1);
 }
 
 hipError_t launch_batch_stage_f64(int stage, const BatchInst* insts, int n_inst, int n_tasks, int resident,

@@ -6,13 +6,18 @@
 // declarations below.
 #pragma once
 #include <algorithm>
+#include <hip/hip_ext.h>
 #include "amhd_kernels.hpp"
 
 namespace amhd {
 
-// One SSP-RK3 stage (0 = RHS only) of the fused strip-march kernel.
-hipError_t launch_stage_f64(int stage, bool powell, bool glm, const StageParams<double>& a, hipStream_t s);
-hipError_t launch_stage_f32(int stage, bool powell, bool glm, const StageParams<float>& a, hipStream_t s);
+// One SSP-RK3 stage (0 = RHS only) of the fused strip-march kernel.  With
+// ev0 and ev1 (created events), the launch records the kernel's own start
+// and end in them (the dispatch's timestamps).
+hipError_t launch_stage_f64(int stage, bool powell, bool glm, const StageParams<double>& a, hipStream_t s,
+                            hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
+hipError_t launch_stage_f32(int stage, bool powell, bool glm, const StageParams<float>& a, hipStream_t s,
+                            hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 
 // Batched stage launch over a device array of BatchInst (persistent grid of
 // `resident` wavefronts).
@@ -56,27 +61,33 @@ inline int grid_for(long n, int block, int cap = 4096) {
 // Template bodies, included by the two precision TUs only.
 #ifdef AMHD_LAUNCH_IMPL
 template <class real, class P, int STAGE>
-void launch_stage_t(bool powell, bool glm, const StageParams<real>& a, dim3 grid, dim3 block, hipStream_t s) {
+void launch_stage_t(bool powell, bool glm, const StageParams<real>& a, dim3 grid, dim3 block, hipStream_t s,
+                    hipEvent_t ev0, hipEvent_t ev1) {
+    auto go = [&](auto kernel) {
+        if (ev0 || ev1) hipExtLaunchKernelGGL(kernel, grid, block, 0, s, ev0, ev1, 0, a);
+        else kernel<<<grid, block, 0, s>>>(a);
+    };
     if (powell) {
-        if (glm) stage_kernel<real, P, STAGE, true, true><<<grid, block, 0, s>>>(a);
-        else stage_kernel<real, P, STAGE, true, false><<<grid, block, 0, s>>>(a);
+        if (glm) go(stage_kernel<real, P, STAGE, true, true>);
+        else go(stage_kernel<real, P, STAGE, true, false>);
     } else {
-        if (glm) stage_kernel<real, P, STAGE, false, true><<<grid, block, 0, s>>>(a);
-        else stage_kernel<real, P, STAGE, false, false><<<grid, block, 0, s>>>(a);
+        if (glm) go(stage_kernel<real, P, STAGE, false, true>);
+        else go(stage_kernel<real, P, STAGE, false, false>);
     }
 }
 
 // glm: the caller guarantees ch > 0 for this launch (the device loop always
 // has ch >= 1.5; host-driven steps pass their ch).
 template <class real, class P>
-hipError_t launch_stage_impl(int stage, bool powell, bool glm, const StageParams<real>& a, hipStream_t s) {
+hipError_t launch_stage_impl(int stage, bool powell, bool glm, const StageParams<real>& a, hipStream_t s,
+                             hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr) {
     const int n_waves = stage_wave_count(a);
     const dim3 grid((n_waves + WPB - 1) / WPB + (a.side ? 1 : 0)), block(WAVE * WPB);
     switch (stage) {
-        case 0: launch_stage_t<real, P, 0>(powell, glm, a, grid, block, s); break;
-        case 1: launch_stage_t<real, P, 1>(powell, glm, a, grid, block, s); break;
-        case 2: launch_stage_t<real, P, 2>(powell, glm, a, grid, block, s); break;
-        case 3: launch_stage_t<real, P, 3>(powell, glm, a, grid, block, s); break;
+        case 0: launch_stage_t<real, P, 0>(powell, glm, a, grid, block, s, ev0, ev1); break;
+        case 1: launch_stage_t<real, P, 1>(powell, glm, a, grid, block, s, ev0, ev1); break;
+        case 2: launch_stage_t<real, P, 2>(powell, glm, a, grid, block, s, ev0, ev1); break;
+        case 3: launch_stage_t<real, P, 3>(powell, glm, a, grid, block, s, ev0, ev1); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

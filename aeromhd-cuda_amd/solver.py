@@ -136,7 +136,7 @@ class Context:
         return tt.value, ss.value, ld.value
 
     def run_profiled(self, n_steps: int, cfl: float, t_end: float, max_steps: int, t: float, step: int):
-        """Device loop with HIP events around every launch; returns
+        """Device loop with HIP events filled by every launch's dispatch; returns
         (t, step, last_dt, [stage-1/2/3 ms], ctl ms) summed over the steps."""
         tt, ss, ld = C.c_double(t), C.c_int(step), C.c_double(0.0)
         ms = (C.c_double * 3)()

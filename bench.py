@@ -10,8 +10,9 @@ timed region.
 
 Prints ONE JSON line: Mcell-updates/s, the roofline of the fused stage
 kernel (algorithmic bytes 8 * 10 * sizeof(real) per cell-update divided by
-the stage kernels' device time, HIP events on the context stream over the
-timed region, replayed from the state the timed steps start from), and the
+the stage kernels' device time, HIP events filled by the stage dispatches on
+the context stream over the timed steps, replayed from the state they start
+from), and the
 CPU oracle timed on this host on a bounded sample.
 Secondary records: the 1024x512 RMI in FP64 and FP32, the batched form (64
 instances), the x-slab decomposition (4 slabs with halo exchange, one
@@ -90,8 +91,8 @@ def gpu_measure(name: str, steps: int, warmup: int, seed: int, rows: int | None,
     assert n == steps, "device loop stopped early"
     t_graph = s.t
     # timed region 2 (roofline): the same steps again from the same state,
-    # launched without the graph, with a HIP event pair around every launch
-    # on the context stream
+    # launched without the graph, each launch on the context stream carrying
+    # a HIP event pair that its dispatch fills with the kernel's start and end
     s.ctx.upload(snap)
     s.t, s.step = t_snap, step_snap
     del snap

@@ -128,11 +128,11 @@ int amhd_synchronize(void* ctx);
 int amhd_run(void* ctx, int n_steps, double cfl, double t_end, int max_steps, int use_graph, double* t_inout,
              int* step_inout, double* last_dt);
 
-/* amhd_run without graph replay and with a HIP event pair around every
- * launch on the context stream: stage_ms[s] receives the summed device time
- * of the stage-(s+1) kernel over the steps run; *ctl_ms (may be NULL)
- * receives 0, the device loop having no separate step-control launch.  For
- * measurement. */
+/* amhd_run without graph replay, every launch on the context stream
+ * carrying a HIP event pair that its dispatch fills with the kernel's start
+ * and end: stage_ms[s] receives the summed device time of the stage-(s+1)
+ * kernel over the steps run; *ctl_ms (may be NULL) receives 0, the device
+ * loop having no separate step-control launch.  For measurement. */
 int amhd_run_profiled(void* ctx, int n_steps, double cfl, double t_end, int max_steps, double* t_inout,
                       int* step_inout, double* last_dt, double stage_ms[3], double* ctl_ms);
 
