@@ -718,15 +718,21 @@ int amhd_set_balance(void* ctx, int on) {
 }
 
 #ifdef AMHD_WAVE_TIMES
-// Diagnostic builds only: the context's segment table (n_segs + 1 ints) and
-// row flags (nx * RB_FLAG_PITCH bytes) copied to host memory.  Not part of
-// the ABI of include/aeromhd.h.
-int amhd_debug_tables(void* ctx, int* seg_i0, unsigned char* flags, int* n_segs) {
-    if (!ctx || !seg_i0 || !flags || !n_segs) return fail(-1, "amhd_debug_tables: null argument");
+// Diagnostic builds only: the context's segment tables (each n_segs + 1
+// ints: the one stage 1 reads, the one the device loop's stages 2 and 3
+// read, the next step's first-stage table) and row flags (nx *
+// RB_FLAG_PITCH bytes) copied to host memory.  Not part of the ABI of
+// include/aeromhd.h.
+int amhd_debug_tables(void* ctx, int* seg_i0, int* seg_next, int* seg_first, unsigned char* flags, int* n_segs) {
+    if (!ctx || !seg_i0 || !seg_next || !seg_first || !flags || !n_segs)
+        return fail(-1, "amhd_debug_tables: null argument");
     Ctx* c = (Ctx*)ctx;
     HIP_TRY(hipStreamSynchronize(c->stream));
     *n_segs = c->n_segs;
-    HIP_TRY(hipMemcpy(seg_i0, c->seg_i0, sizeof(int) * (size_t)(c->n_segs + 1), hipMemcpyDeviceToHost));
+    const size_t tb = sizeof(int) * (size_t)(c->n_segs + 1);
+    HIP_TRY(hipMemcpy(seg_i0, c->seg_i0, tb, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(seg_next, c->seg_next, tb, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(seg_first, c->seg_first, tb, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(flags, c->rowflags, (size_t)RB_FLAG_PITCH * c->nx, hipMemcpyDeviceToHost));
     return 0;
 }

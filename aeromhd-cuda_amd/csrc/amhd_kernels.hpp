@@ -127,7 +127,7 @@ struct Rebalance {
 // three states per cell, and such a row costs about what its memory
 // traffic does.  FP64 is bound by its arithmetic in every stage.
 #ifndef AMHD_RB_W1
-#define AMHD_RB_W1 3, 2, 4
+#define AMHD_RB_W1 3, 2, 3
 #endif
 #ifndef AMHD_RB_W23
 #define AMHD_RB_W23 4, 2, 1
