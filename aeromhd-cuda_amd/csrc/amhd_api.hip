@@ -317,6 +317,7 @@ hipError_t launch_loop_stage(Ctx* c, int stage, const void* u0, const void* us, 
         a.ctl = c->ctl;
         a.ctl_out = c->ctl_next;
         a.part_in = c->part_dt;
+        a.n_part_in = loop_parts(c);
         a.seg_i0 = c->seg_i0;
         if (bal) {
             // this step's table for stages 2 and 3, the next step's for stage 1
@@ -380,6 +381,31 @@ int ensure_partials(Ctx* c) {
                                    c->part_dt, c->part_cap, c->stream);
     if (g < 0) return fail(-2, "dt partials launch failed: %s", hipGetErrorString(hipGetLastError()));
     c->n_part = g;
+    c->part_valid = true;
+    return 0;
+}
+
+// The device loop's first stage reduces loop_parts(c) partials (a launch
+// constant, so that no wavefront waits on a count before loading them):
+// the partials of the current state in exactly that many slots, the
+// unused ones 0 (neutral for maxima of non-negative terms).
+int ensure_loop_partials(Ctx* c) {
+    const int nw = loop_parts(c);
+    if (c->part_valid && c->n_part == nw) return 0;
+    int g;
+    if (c->dtype == AMHD_F64)
+        g = launch_dt_partials_f64((const double*)c->buf[c->cur], c->plane, c->pitch, c->nx, c->ny, c->p.gamma,
+                                   c->part_dt, nw, c->stream);
+    else
+        g = launch_dt_partials_f32((const float*)c->buf[c->cur], c->plane, c->pitch, c->nx, c->ny, c->p.gamma,
+                                   c->part_dt, nw, c->stream);
+    if (g < 0) return fail(-2, "dt partials launch failed: %s", hipGetErrorString(hipGetLastError()));
+    if (g < nw) {
+        const hipError_t e = hipMemsetAsync(c->part_dt + (size_t)g * NPART_DT, 0, sizeof(double) * NPART_DT * (nw - g),
+                                            c->stream);
+        if (e != hipSuccess) return fail(-2, "partials fill failed: %s", hipGetErrorString(e));
+    }
+    c->n_part = nw;
     c->part_valid = true;
     return 0;
 }
@@ -908,7 +934,7 @@ int amhd_run(void* ctx, int n_steps, double cfl, double t_end, int max_steps, in
     h.min_dxdy = std::min(c->p.dx, c->p.dy);
     h.done = 0;
     h.pending = 0;
-    int rc = ensure_partials(c);
+    int rc = ensure_loop_partials(c);
     if (rc) return rc;
     h.n_part = c->n_part;   // the first stage reduces these partials
     HIP_TRY(hipMemcpyAsync(c->ctl, &h, sizeof h, hipMemcpyHostToDevice, c->stream));
@@ -958,7 +984,7 @@ int amhd_run_profiled(void* ctx, int n_steps, double cfl, double t_end, int max_
     h.max_steps = (int)std::min<long>((long)max_steps, (long)start + n_steps);
     h.glm_alpha = c->p.glm_alpha; h.min_dxdy = std::min(c->p.dx, c->p.dy);
     h.done = 0; h.pending = 0;
-    int rc = ensure_partials(c);
+    int rc = ensure_loop_partials(c);
     if (rc) return rc;
     h.n_part = c->n_part;
     HIP_TRY(hipMemcpyAsync(c->ctl, &h, sizeof h, hipMemcpyHostToDevice, c->stream));

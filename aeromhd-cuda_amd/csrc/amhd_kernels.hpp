@@ -185,6 +185,7 @@ template <class real> struct StageParams {
     // ctl_out (the committed block) and copies seg_copy_src to seg_copy.
     int fuse, side;
     const double* part_in;
+    int n_part_in;   // STAGE 1 fused: the CFL partials in part_in, at least 1 (a launch constant of the device loop)
     StepCtl* ctl_out;
     const double* part_bf_in;
     int n_part_next;   // STAGE 3 side: the CFL partials the stage writes (one per strip workgroup)
@@ -429,37 +430,52 @@ __device__ __forceinline__ double uniform_d(double x) {
 // CFL partials itself and begins the step from the committed control block
 // the way the step control kernel does (the same maxima and expressions, so
 // every wavefront holds the same values); wavefront 0 of workgroup 0 stores
-// the step's block.  First half: this lane's maxima over partials lane,
-// lane + 64, ... (eight in flight), issued before the segment's first rows
-// so that both latencies overlap.
-__device__ __forceinline__ void lane_cfl_maxima(const double* __restrict__ part, int n_part, int lane,
-                                                double (&m)[NPART_DT]) {
-    constexpr int U = 8;
+// the step's block.  First half, in two parts so that the partials' loads
+// and the segment's first rows are in flight together: cfl_issue loads this
+// lane's partials lane, lane + 64, ..., lane + 448 (an index past the last
+// reads the last partial again, which leaves the maxima unchanged, so the
+// loads need no test), and cfl_reduce, after the rows are requested, takes
+// their maxima and those of any partials past the first 512.
+constexpr int CFL_U = 8;
+__device__ __forceinline__ void cfl_issue(const double* __restrict__ part, int n_part, int lane,
+                                          double (&v)[CFL_U][NPART_DT]) {
+#pragma unroll
+    for (int u = 0; u < CFL_U; ++u) {
+        const int k = min(lane + u * WAVE, n_part - 1);
+#pragma unroll
+        for (int q = 0; q < NPART_DT; ++q) v[u][q] = part[(long)k * NPART_DT + q];
+    }
+}
+__device__ __forceinline__ void cfl_reduce(const double* __restrict__ part, int n_part, int lane,
+                                           double (&v)[CFL_U][NPART_DT], double (&m)[NPART_DT]) {
+    // an empty statement that takes the partials and clobbers memory: the
+    // maxima below cannot be placed before the loads issued ahead of this
+    // point (the compiler otherwise waits on the partials before issuing them)
+#pragma unroll
+    for (int u = 0; u < CFL_U; ++u) {
+        asm volatile("" : "+v"(v[u][0]), "+v"(v[u][1]), "+v"(v[u][2]), "+v"(v[u][3])::"memory");
+    }
 #pragma unroll
     for (int q = 0; q < NPART_DT; ++q) m[q] = 0.0;
-    for (int k0 = lane; k0 < n_part; k0 += WAVE * U) {
-        double v[U][NPART_DT];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int k = k0 + u * WAVE;
+    for (int u = 0; u < CFL_U; ++u) {
 #pragma unroll
-            for (int q = 0; q < NPART_DT; ++q) v[u][q] = k < n_part ? part[(long)k * NPART_DT + q] : 0.0;
-        }
+        for (int q = 0; q < NPART_DT; ++q) m[q] = fmax(m[q], v[u][q]);
+    }
+    for (int k = lane + CFL_U * WAVE; k < n_part; k += WAVE) {
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-#pragma unroll
-            for (int q = 0; q < NPART_DT; ++q) m[q] = fmax(m[q], v[u][q]);
-        }
+        for (int q = 0; q < NPART_DT; ++q) m[q] = fmax(m[q], part[(long)k * NPART_DT + q]);
     }
 }
 // Second half: the wave's maxima and the step values.  Returns false when
 // the loop is done (no step runs).
 template <class real>
-__device__ __forceinline__ bool fused_begin(const StageParams<real>& a, const double (&m)[NPART_DT], CtlVals& cv) {
+__device__ __forceinline__ bool fused_begin(const StageParams<real>& a, const double (&m)[NPART_DT], const StepCtl& h0,
+                                           CtlVals& cv) {
     double r[NPART_DT];
 #pragma unroll
     for (int q = 0; q < NPART_DT; ++q) r[q] = wave_max(m[q]);
-    StepCtl h = *a.ctl;
+    StepCtl h = h0;
     if (!h.done) ctl_begin(h, ctl_dt(h, r, a.dx, a.dy, h.cfl));
     if (blockIdx.x == 0 && threadIdx.x == 0) *a.ctl_out = h;
     cv.dt = uniform_d(h.dt);
@@ -547,7 +563,8 @@ __device__ __forceinline__ void stage_wave(const StageParams<real>& a, const int
 
     // The step's dt, ch and psi decay: launch arguments, the device loop's
     // control block, or (fused first stage) reduced by the wavefront beside
-    // the segment's first rows (lane_cfl_maxima, fused_begin in the prologue)
+    // the segment's first rows (cfl_issue, cfl_reduce, fused_begin in the
+    // prologue).
     const bool fused = STAGE == 1 && a.fuse;
     double dt_d = a.dt, ch_d = a.ch, decay_d = a.decay;
     if (a.ctl && !fused) {
@@ -570,7 +587,7 @@ __device__ __forceinline__ void stage_wave(const StageParams<real>& a, const int
         cx = real(wst * dt_d / a.dx); cy = real(wst * dt_d / a.dy);
         pcx = real(0.5 * wst * dt_d / a.dx); pcy = real(0.5 * wst * dt_d / a.dy);
     };
-    set_step(dt_d, ch_d, decay_d);
+    if (!fused) set_step(dt_d, ch_d, decay_d);
 
     constexpr unsigned ES = sizeof(real);
     const unsigned pb = (unsigned)a.plane * ES;             // plane stride, bytes
@@ -643,15 +660,20 @@ __device__ __forceinline__ void stage_wave(const StageParams<real>& a, const int
         // fused first stage: the CFL partials (cache-resident) are loaded
         // first and reduced while the segment's rows, loaded next, are on
         // their way from memory
-        double pm[NPART_DT];
-        if (fused) lane_cfl_maxima(a.part_in, a.ctl->n_part, lane, pm);
+        double pv[CFL_U][NPART_DT], pm[NPART_DT];
+        if (fused) cfl_issue(a.part_in, a.n_part_in, lane, pv);
         load_in(i0 - 2, um2);
         load_in(i0 - 1, um1);
         load_in(i0, QA);
         load_in(i0 + 1, QB);
+        if (fused) cfl_reduce(a.part_in, a.n_part_in, lane, pv, pm);
         if (fused) {
+            // (scalar loads return out of order, so a wait on any of them
+            // waits on all: the control block is loaded after the rows are
+            // requested)
+            const StepCtl hc = *a.ctl;
             CtlVals v;
-            if (!fused_begin(a, pm, v)) return;   // the loop is done
+            if (!fused_begin(a, pm, hc, v)) return;   // the loop is done
             set_step(v.dt, v.ch, v.decay);
         }
         cons_to_prim<real, P>(um2, wm2, kc.gamma, psi_ghost(i0 - 2));
