@@ -569,15 +569,6 @@ __device__ __forceinline__ void phys_flux_x(const real (&w)[NV], real (&f)[NV], 
     f[9] = m * w[iCLR];
 }
 
-// True when every component of d is +0 or -0 (NaN and inf are not).
-template <class real>
-__device__ __forceinline__ bool all_zero(const real (&d)[NV]) {
-    unsigned acc = 0u;
-#pragma unroll
-    for (int v = 0; v < NV; ++v) acc |= __float_as_uint(d[v]);
-    return (acc & 0x7fffffffu) == 0u;
-}
-
 // True when a and b hold the same bits in every component.  A computation
 // fed with the same bits returns the same bits, so a face whose inputs are
 // those of a face already solved may take that face's result.
