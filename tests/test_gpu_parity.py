@@ -540,7 +540,8 @@ def test_graph_and_host_loops_agree(torch_gpu):
     assert out[0][1:] == out[1][1:]
 
 
-@pytest.mark.parametrize("precision,nx,ny", [("fp32", 256, 128), ("fp64", 128, 64)])
+@pytest.mark.parametrize("precision,nx,ny", [("fp32", 256, 128), ("fp64", 128, 64), ("fp32", 512, 8),
+                                             ("fp64", 512, 8)])
 def test_device_loop_equals_host_steps_to_t_end(torch_gpu, precision, nx, ny):
     """The device loop, whose first and third stage kernels carry the step
     control (CFL maxima reduced at the start of the first stage, the step
@@ -550,7 +551,9 @@ def test_device_loop_equals_host_steps_to_t_end(torch_gpu, precision, nx, ny):
     the same state, t, step count, last dt and boundary energy bit for bit.
     FP64: the host-driven steps take the psi decay from the host's exp, the
     device loop from the device's, which may differ in the last bit; the
-    step count is equal and everything else within 1e-13."""
+    step count is equal and everything else within 1e-13.  At 512 x 8 the
+    state's CFL partials fill fewer slots than the loop's first stage reads
+    (the rest are zeros)."""
     from amhd_ics import cons_from_prim_host, magnetic_sieve
     from solver import MHDSolver
 
