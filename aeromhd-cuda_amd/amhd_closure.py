@@ -33,8 +33,15 @@ smaller theta of its two cells, so the limited correction is still a flux
 difference.  The corrected state is checked before it is written
 (amhd_closure_check: density and pressure positive, no non-finite values,
 max|divB| not above 1.5x its pre-step value plus a floor); a correction
-that fails is not applied, and the plain coarse step stands.  BatchClosure
-does the same for many instances in one batched step.
+that fails is not applied, and the plain coarse step stands.  After an
+accepted correction, amhd_closure_sharpen moves rhoC across the x faces
+against the gradient of C (limited anti-diffusion, flux-corrected
+transport with the pre- and post-step states' bounds) with the coefficient
+stats["kappa"], calibrated on training seeds after the gain: the plain
+coarse step smears the interface across x faces faster than the limited
+mixing flux can undo.  The same pass without the network (_Sharpener,
+stats["kappa_alone"]) is evaluated beside it.  BatchClosure does all of
+this for many instances in one batched step.
 
 The fine-grid solver is the reference: integral mixing width and L1
 density error of the coarse run against the restricted fine run.
@@ -508,9 +515,12 @@ class _Corrector:
     on the pre-step states, limited source against the post-step states,
     the guard evaluated on U + dt S before anything is written, and
     U += dt S for the instances that pass (the others keep their plain
-    step: the fallback).  All launches on `stream`."""
+    step: the fallback), then, on those, the limited anti-diffusion of C
+    across x faces (amhd_closure_sharpen) with the coefficient kappa.  All
+    launches on `stream`."""
 
-    def __init__(self, model: FluxCNN, n: int, cfg, dtype: int, stream, keep, divb_floor, fields, gain):
+    def __init__(self, model: FluxCNN, n: int, cfg, dtype: int, stream, keep, divb_floor, fields, gain,
+                 kappa=None):
         torch = _L.require_gpu()
         self.model, self.n, self.cfg, self.dtype, self.stream = model, n, cfg, dtype, stream
         self.keep, self.divb_floor = keep, divb_floor
@@ -521,6 +531,11 @@ class _Corrector:
         if gain is None:
             gain = float(np.asarray(model.stats.get("gain", 1.0)))
         self.mask *= np.float32(gain)
+        if kappa is None:
+            kappa = float(np.asarray(model.stats.get("kappa", 0.0)))
+        if not (kappa >= 0.0 and np.isfinite(kappa)):
+            raise ValueError(f"kappa must be finite and >= 0, got {kappa}")
+        self.kappa = float(kappa)
         nx, ny = cfg.nx, cfg.ny
         tdt = torch.float32 if dtype == _L.AMHD_F32 else torch.float64
         self.x = torch.empty(n, nx, ny, 16, device="cuda", dtype=torch.bfloat16)
@@ -528,6 +543,8 @@ class _Corrector:
         self.theta = torch.ones(2 * n * nx * ny, device="cuda", dtype=torch.float32)   # theta, theta_mix
         self.src = torch.zeros(n, 10, nx, ny, device="cuda", dtype=tdt)
         self.work = torch.empty(32 * 64 * 4, device="cuda", dtype=torch.float64)
+        self.sharp_work = (torch.empty(3 * min(n, 32) * nx * ny, device="cuda", dtype=torch.float64)
+                           if self.kappa > 0 else None)
         step = 10 * nx * ny * self.src.element_size()
         self.src_ptrs = [self.src.data_ptr() + m * step for m in range(n)]
         torch.cuda.synchronize()
@@ -581,7 +598,8 @@ class _Corrector:
 
     def correct(self, pre, post, dt, div0):
         """Source against the post-step states (bounds also from the pre-step
-        ones), guard, apply; returns (accepted mask, |divB| of the states now
+        ones), guard, apply, sharpen C on the accepted states (bounds again
+        from both states); returns (accepted mask, |divB| of the states now
         current)."""
         self._plan()
         cfg, n = self.cfg, self.n
@@ -603,10 +621,44 @@ class _Corrector:
             _L.check(lib.amhd_closure_apply(W(*[post[i] for i in idx]), W(*[self.src_ptrs[i] for i in idx]),
                                             _dvec([dt[i] for i in idx]), self.dtype, k, cfg.nx, cfg.ny, self.stream),
                      "amhd_closure_apply")
+            if self.kappa > 0:
+                _L.check(lib.amhd_closure_sharpen(W(*[post[i] for i in idx]), W(*[pre[i] for i in idx]), self.dtype,
+                                                  k, cfg.nx, cfg.ny, self.kappa, _p(self.sharp_work), self.stream),
+                         "amhd_closure_sharpen")
         div_now = r[:, 2].copy()
         if not ok.all():
             div_now[~ok] = self.check(post, np.zeros(n))[~ok, 2]
         return ok, div_now
+
+
+class _Sharpener:
+    """The limited anti-diffusion of C across x faces without the network:
+    after each plain step, amhd_closure_sharpen on the post-step states with
+    the pre-step states' bounds.  The interface of _Corrector; there is no
+    guard to take (only rhoC moves, and C stays within its bounds)."""
+
+    def __init__(self, n: int, cfg, dtype: int, stream, kappa: float):
+        torch = _L.require_gpu()
+        if not (kappa > 0.0 and np.isfinite(kappa)):
+            raise ValueError(f"the sharpening needs a finite kappa > 0, got {kappa}")
+        self.lib = _L.load()
+        self.n, self.cfg, self.dtype, self.stream, self.kappa = n, cfg, dtype, stream, float(kappa)
+        self.work = torch.empty(3 * min(n, 32) * cfg.nx * cfg.ny, device="cuda", dtype=torch.float64)
+
+    def predict(self, pre):
+        pass
+
+    def check(self, ptrs, dt) -> np.ndarray:
+        return np.zeros((len(ptrs), 4))
+
+    def correct(self, pre, post, dt, div0):
+        idx = [i for i in range(len(post)) if dt[i] > 0]
+        if idx:
+            V = C.c_void_p * len(idx)
+            _L.check(self.lib.amhd_closure_sharpen(V(*[post[i] for i in idx]), V(*[pre[i] for i in idx]), self.dtype,
+                                                   len(idx), self.cfg.nx, self.cfg.ny, self.kappa, _p(self.work),
+                                                   self.stream), "amhd_closure_sharpen")
+        return np.ones(len(post), dtype=bool), np.asarray(div0, dtype=np.float64)
 
 
 def _dvec(v):
@@ -630,20 +682,24 @@ class ClosureSolver:
     context's stream and reads the state in place."""
 
     def __init__(self, cfg, model: FluxCNN | None, divb_floor: float = 1.0, keep: float | None = 0.5,
-                 fields=None, gain: float | None = None):
+                 fields=None, gain: float | None = None, kappa: float | None = None):
         """fields: the network fields (positions in FEAT) whose corrections
         are applied, None for the model's own (all unless it was trained
         on fewer); the others keep the plain solver's fluxes.  gain: a
         factor on the corrections, None for the model's calibrated one
         (stats["gain"], 1 if absent).  keep: the positivity limiter's bound
-        (None: no limiters)."""
+        (None: no limiters).  kappa: the coefficient of the limited
+        anti-diffusion of C across x faces after each accepted correction,
+        None for the model's calibrated one (stats["kappa"], 0 -- none --
+        if absent).  model None: the plain solver, followed by that
+        anti-diffusion alone when kappa > 0."""
         from solver import MHDSolver
 
         check_closure_boundaries(cfg)
         self.s = MHDSolver(cfg)
         self.model = model
         self.fallbacks = 0
-        self._opts = (keep, divb_floor, fields, gain)
+        self._opts = (keep, divb_floor, fields, gain, kappa)
         self.cor = None
 
     def check(self):
@@ -660,11 +716,13 @@ class ClosureSolver:
     def begin(self):
         """Upload an assigned state and take the guard's reference |divB|."""
         self.s._sync_state()
-        if self.model is not None and self.cor is None:
+        kappa = self._opts[4]
+        if self.cor is None and (self.model is not None or (kappa is not None and kappa > 0)):
             ctx = self.s.ctx
             st = C.c_void_p()
             _L.check(ctx.lib.amhd_get_stream(ctx.h, C.byref(st)), "amhd_get_stream")
-            self.cor = _Corrector(self.model, 1, self.s.cfg, ctx.dtype, st, *self._opts)
+            self.cor = (_Corrector(self.model, 1, self.s.cfg, ctx.dtype, st, *self._opts) if self.model is not None
+                        else _Sharpener(1, self.s.cfg, ctx.dtype, st, kappa))
         self.div0 = self.check()[2]
 
     def step(self, dt: float | None = None, on_plain=None) -> bool:
@@ -676,8 +734,8 @@ class ClosureSolver:
         on_plain() runs between the plain step and the correction.  Returns
         whether the correction was applied, or None when no step ran."""
         s = self.s
-        pre = self._state_ptr() if self.model is not None else None
-        if self.model is not None:
+        pre = self._state_ptr() if self.cor is not None else None
+        if self.cor is not None:
             self.cor.predict([pre])
         if dt is None:
             if s.advance(1, use_graph=False) == 0:   # one step: a 3-step graph would replay two no-ops
@@ -689,7 +747,7 @@ class ClosureSolver:
             s.step_ssprk3()
             s.t += dt
             s.step += 1
-        if self.model is None:
+        if self.cor is None:
             return True
         post = self._state_ptr()
         if on_plain is not None:
@@ -720,7 +778,7 @@ class BatchClosure:
     as ClosureSolver.run would step it alone."""
 
     def __init__(self, cases, model: FluxCNN | None, divb_floor: float = 1.0, keep: float | None = 0.5,
-                 fields=None, gain: float | None = None):
+                 fields=None, gain: float | None = None, kappa: float | None = None):
         torch = _L.require_gpu()
         from amhd_batch import BatchSolver
 
@@ -738,8 +796,12 @@ class BatchClosure:
         # the end of each step waits for that stream itself
         self.tstream = torch.cuda.current_stream()
         self.stream = C.c_void_p(self.tstream.cuda_stream)
-        self.cor = (_Corrector(model, self.n, cfgs[0], dtype, self.stream, keep, divb_floor, fields, gain)
-                    if model is not None else None)
+        if model is not None:
+            self.cor = _Corrector(model, self.n, cfgs[0], dtype, self.stream, keep, divb_floor, fields, gain, kappa)
+        elif kappa is not None and kappa > 0:
+            self.cor = _Sharpener(self.n, cfgs[0], dtype, self.stream, kappa)
+        else:
+            self.cor = None
 
     def _ptrs(self):
         n = self.n
@@ -781,14 +843,28 @@ class BatchClosure:
 
 # ------------------------------------------------------- evaluation
 VARIANTS = {
-    "closure": dict(keep=0.5),                       # the model's own fields and gain
-    "nolimit": dict(keep=None),                      # ... without the limiters
+    "closure": dict(keep=0.5),                       # the model's own fields, gain and kappa
+    "network": dict(keep=0.5, kappa=0.0),            # ... without the sharpening of C
+    "nolimit": dict(keep=None, kappa=0.0),           # ... and without the limiters
+    # the sharpening alone (no network pass) at its own calibrated
+    # coefficient (stats["kappa_alone"])
+    "sharpen": dict(network=False, kappa="alone"),
     # ablations of the network itself (gain 1)
-    "all": dict(keep=0.5, fields=list(range(N_IN)), gain=1.0),
-    "mix": dict(keep=0.5, fields=[RC], gain=1.0),
-    "mass": dict(keep=0.5, fields=[0, RC], gain=1.0),
-    "hydro": dict(keep=0.5, fields=[0, 1, 2, 3, 7, RC], gain=1.0),
+    "all": dict(keep=0.5, fields=list(range(N_IN)), gain=1.0, kappa=0.0),
+    "mix": dict(keep=0.5, fields=[RC], gain=1.0, kappa=0.0),
+    "mass": dict(keep=0.5, fields=[0, RC], gain=1.0, kappa=0.0),
+    "hydro": dict(keep=0.5, fields=[0, 1, 2, 3, 7, RC], gain=1.0, kappa=0.0),
 }
+
+
+def _resolve(model, kw):
+    """(model or None, ClosureSolver keywords) of a variant: kappa="alone"
+    becomes the model's calibrated coefficient of the sharpening alone, and
+    network=False drops the model."""
+    kw = dict(kw)
+    if kw.get("kappa") == "alone":
+        kw["kappa"] = float(np.asarray(model.stats.get("kappa_alone", 0.0)))
+    return (model if kw.pop("network", True) else None), kw
 
 
 def evaluate(seeds, model, steps: int = 2000, nx: int = 2048, ny: int = 1024, log=None, variants=("closure",),
@@ -797,8 +873,9 @@ def evaluate(seeds, model, steps: int = 2000, nx: int = 2048, ny: int = 1024, lo
     (VARIANTS: limiter on/off, which corrections apply) from the restricted
     initial condition to the fine run's final time.  Errors are taken
     against the restricted fine state (the coarse grid's best attainable
-    answer): the integral mixing width of the restricted state (mw_err_*)
-    and the L1 density error (l1rho_*); the fine grid's own mixing width is
+    answer): the integral mixing width of the restricted state (mw_err_*),
+    the L1 density error (l1rho_*) and the mean |C - C_fine| (l1c_*, C
+    clipped to [0, 1]); the fine grid's own mixing width is
     reported as mw_fine.  specs: {tag: ClosureSolver keywords} in place of
     VARIANTS."""
     torch = _L.require_gpu()
@@ -826,7 +903,10 @@ def evaluate(seeds, model, steps: int = 2000, nx: int = 2048, ny: int = 1024, lo
         rec = {"seed": seed, "t_end": t_end, "mw_fine": mw_f, "mw_fine_restricted": mw_fr, "fine_wall_s": fine_wall}
         table = specs if specs is not None else VARIANTS
         runs = [("plain", None, {})] + [(v, model, table[v]) for v in variants]
+        c_f = (Ufr[9] / Ufr[0]).clamp(0, 1)
         for tag, m, kw in runs:
+            if m is not None:
+                m, kw = _resolve(m, kw)
             cs = ClosureSolver(ccfg, m, **kw)
             cs.s.U = ghosted(fine["U0"])
             torch.cuda.synchronize()
@@ -839,6 +919,7 @@ def evaluate(seeds, model, steps: int = 2000, nx: int = 2048, ny: int = 1024, lo
             rec[f"mw_{tag}"] = mixing_width(Uc.cpu().numpy(), ccfg)
             rec[f"mw_err_{tag}"] = abs(rec[f"mw_{tag}"] - mw_fr) / max(abs(mw_fr), 1e-30)
             rec[f"l1rho_{tag}"] = float((rho_c - rho_f).abs().mean() / rho_f.abs().mean())
+            rec[f"l1c_{tag}"] = float(((Uc[9, 2:-2, 2:-2] / rho_c).clamp(0, 1) - c_f).abs().mean())
             rec[f"steps_{tag}"] = cs.s.step
             rec[f"wall_{tag}_s"] = wall
             rec[f"fallbacks_{tag}"] = cs.fallbacks
@@ -872,9 +953,10 @@ def fp64_fine_throughput(seed: int, steps: int = 2000, nx: int = 2048, ny: int =
 
 def evaluate_batched(seeds, model, steps: int = 2000, nx: int = 2048, ny: int = 1024, log=None,
                      reps: int = 3) -> dict:
-    """The batched form: the plain coarse solver and the closure each run
-    over all seeds at once (BatchSolver / BatchClosure, every instance to
-    its own fine run's end time), scored per seed as evaluate() does.
+    """The batched form: the plain coarse solver, the closure and (when the
+    model holds its coefficient) the sharpening of C alone each run over all
+    seeds at once (BatchSolver / BatchClosure, every instance to its own
+    fine run's end time), scored per seed as evaluate() does.
 
     Wall time: the median of `reps` timed runs from the same initial states,
     after an untimed run of 2 steps that loads the kernels, as the FP64
@@ -897,16 +979,21 @@ def evaluate_batched(seeds, model, steps: int = 2000, nx: int = 2048, ny: int = 
         refs.append((Ufr, mixing_width(ghosted(Ufr).cpu().numpy(), ccfg)))
         cases.append((ccfg, ghosted(fine["U0"]).cpu().numpy()))
     out = {"seeds": list(seeds), "instances": len(cases)}
-    for tag, m in (("plain", None), ("closure", model)):
+    forms = [("plain", None, {}), ("closure", model, {})]
+    ka = float(np.asarray(model.stats.get("kappa_alone", 0.0))) if model is not None else 0.0
+    if ka > 0:
+        forms.append(("sharpen", None, {"kappa": ka}))
+        out["kappa_alone"] = ka
+    for tag, m, kw in forms:
         warm = []
         for c, U in cases:
             wc = Config(**{f: getattr(c, f) for f in c.__dataclass_fields__})
             wc.max_steps = 2
             warm.append((wc, U))
-        BatchClosure(warm, m).run()
+        BatchClosure(warm, m, **kw).run()
         walls = []
         for _ in range(max(1, reps)):
-            bc = BatchClosure(cases, m)
+            bc = BatchClosure(cases, m, **kw)
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             bc.run()
@@ -914,15 +1001,18 @@ def evaluate_batched(seeds, model, steps: int = 2000, nx: int = 2048, ny: int = 
             walls.append(time.perf_counter() - t0)
         out[f"wall_{tag}_s"] = float(np.median(walls))
         out[f"wall_{tag}_runs_s"] = [round(w, 4) for w in walls]
-        mw_err, l1 = [], []
+        mw_err, l1, l1c = [], [], []
         for k, (Ufr, mw_fr) in enumerate(refs):
             Uc = bc.b.download(k)
             mw_err.append(abs(mixing_width(Uc, cases[k][0]) - mw_fr) / max(abs(mw_fr), 1e-30))
             rho_c = Uc[0, 2:-2, 2:-2]
             rho_f = Ufr[0].cpu().numpy()
             l1.append(float(np.abs(rho_c - rho_f).mean() / np.abs(rho_f).mean()))
+            c_f = np.clip(Ufr[9].cpu().numpy() / rho_f, 0, 1)
+            l1c.append(float(np.abs(np.clip(Uc[9, 2:-2, 2:-2] / rho_c, 0, 1) - c_f).mean()))
         out[f"mw_err_{tag}_mean"] = float(np.mean(mw_err))
         out[f"l1rho_{tag}_mean"] = float(np.mean(l1))
+        out[f"l1c_{tag}_mean"] = float(np.mean(l1c))
         out[f"fallbacks_{tag}"] = int(bc.fallbacks.sum())
         out[f"steps_{tag}"] = int(bc.b.steps.sum())
     if log:
@@ -1011,11 +1101,29 @@ def calibrate_gain(model, seeds, gains, steps: int = 2000, log=None):
     return float(specs[best]["gain"]), err
 
 
+def calibrate_kappa(model, seeds, kappas, steps: int = 2000, log=None, alone: bool = False):
+    """The coefficient of the limited anti-diffusion of C that minimises the
+    mean mixing-width error over `seeds` (training seeds), with the model's
+    corrections at their calibrated gain, or, alone=True, without the
+    network (the sharpening by itself).  Stores it in
+    model.stats["kappa"] (alone: model.stats["kappa_alone"])."""
+    specs = {f"k{k:g}": (dict(network=False, kappa=float(k)) if alone else dict(keep=0.5, kappa=float(k)))
+             for k in kappas}
+    rows = evaluate(seeds, model, steps=steps, variants=tuple(specs), specs=specs)
+    err = {t: sum(r[f"mw_err_{t}"] for r in rows) / len(rows) for t in specs}
+    best = min(err, key=err.get)
+    model.stats["kappa_alone" if alone else "kappa"] = np.asarray(specs[best]["kappa"], dtype=np.float64)
+    if log:
+        log(f"[closure] {'sharpening-alone ' if alone else ''}kappa calibration on seeds {list(seeds)}: " +
+            ", ".join(f"{t}={e:.4f}" for t, e in err.items()) + f" -> {best}")
+    return float(specs[best]["kappa"]), err
+
+
 def summarize(rows, tags) -> dict:
     """Means over the evaluated seeds."""
     out = {}
     for t in tags:
-        for k in ("mw_err", "l1rho", "fallbacks", "steps", "wall"):
+        for k in ("mw_err", "l1rho", "l1c", "fallbacks", "steps", "wall"):
             key = f"wall_{t}_s" if k == "wall" else f"{k}_{t}"
             out[key + "_mean"] = sum(r[key] for r in rows) / len(rows)
     return out

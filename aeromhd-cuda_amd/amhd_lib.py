@@ -91,6 +91,7 @@ SIGNATURES = {
                                  C.POINTER(C.c_float), _d, _d, _dp, _d, _d, _vp, C.POINTER(_vp), _i, _vp]),
     "amhd_closure_check": (_i, [C.POINTER(_vp), C.POINTER(_vp), _dp, _i, _i, _i, _i, _d, _d, _d, _vp, _dp, _vp]),
     "amhd_closure_apply": (_i, [C.POINTER(_vp), C.POINTER(_vp), _dp, _i, _i, _i, _i, _vp]),
+    "amhd_closure_sharpen": (_i, [C.POINTER(_vp), C.POINTER(_vp), _i, _i, _i, _i, _d, _vp, _vp]),
 }
 
 _lib = None

@@ -678,6 +678,71 @@ __global__ __launch_bounds__(256) void closure_apply_kernel(MutPtrs st, ConstPtr
     for (int v = 0; v < 10; ++v) u[v * plane + c] = real(double(u[v * plane + c]) + sd.dt[m] * double(s[v * plane + c]));
 }
 
+// Limited anti-diffusion of C across x faces (flux-corrected transport,
+// Zalesak 1979), pass 1 of 2: per cell, the anti-diffusive rhoC flux of its
+// face i+1/2, A = kappa (rho_i + rho_{i+1}) / 2 (C_{i+1} - C_i) (0 on the
+// last column), and the ratios R+ = min(1, Q+ / P+), R- = min(1, Q- / P-)
+// of the room left below C_hi and above C_lo (the min / max of C over the
+// cell and its face neighbours, in the state and, when has_b, in the state
+// before the step) to the cell's inflows and outflows over its two x faces.
+// work holds three planes of cnt * nx * ny doubles: R+, R-, A.
+template <class real>
+__global__ __launch_bounds__(256) void sharpen_ratios_kernel(ConstPtrs st, ConstPtrs bst, bool has_b, int n, int nx,
+                                                             int ny, double kappa, double* __restrict__ work) {
+    const long plane = (long)nx * ny;
+    const long total = (long)n * plane;
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= total) return;
+    const int m = (int)(k / plane);
+    const long c = k - (long)m * plane;
+    const int i = (int)(c / ny), j = (int)(c - (long)i * ny);
+    const real* u = (const real*)st.p[m];
+    const Nbr q = neighbours(u, plane, c, i, j, nx, ny);
+    const double r = double(u[c]);
+    const double al = i > 0 ? kappa * 0.5 * (double(u[c - ny]) + r) * (q.c - q.xl) : 0.0;
+    const double ar = i + 1 < nx ? kappa * 0.5 * (r + double(u[c + ny])) * (q.xr - q.c) : 0.0;
+    const double gin = fmax(al, 0.0) + fmax(-ar, 0.0);
+    const double gout = fmax(ar, 0.0) + fmax(-al, 0.0);
+    double lo = fmin(q.c, fmin(fmin(q.xl, q.xr), fmin(q.yl, q.yr)));
+    double hi = fmax(q.c, fmax(fmax(q.xl, q.xr), fmax(q.yl, q.yr)));
+    if (has_b) {
+        const Nbr b = neighbours((const real*)bst.p[m], plane, c, i, j, nx, ny);
+        lo = fmin(lo, fmin(b.c, fmin(fmin(b.xl, b.xr), fmin(b.yl, b.yr))));
+        hi = fmax(hi, fmax(b.c, fmax(fmax(b.xl, b.xr), fmax(b.yl, b.yr))));
+    }
+    const double rc = double(u[9 * plane + c]);
+    const double qp = fmax(r * hi - rc, 0.0), qm = fmax(rc - r * lo, 0.0);
+    work[k] = gin > 0.0 ? fmin(1.0, qp / gin) : 0.0;
+    work[total + k] = gout > 0.0 ? fmin(1.0, qm / gout) : 0.0;
+    work[2 * total + k] = ar;
+}
+
+// Pass 2: rhoC_i += L(i-1/2) - L(i+1/2), L = A * min(R+ of the receiving
+// cell, R- of the giving one).  Reads only the work planes and the cell's
+// own rhoC, so no cell reads what another writes.
+template <class real>
+__global__ __launch_bounds__(256) void sharpen_apply_kernel(MutPtrs st, int n, int nx, int ny,
+                                                            const double* __restrict__ work) {
+    const long plane = (long)nx * ny;
+    const long total = (long)n * plane;
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= total) return;
+    const int m = (int)(k / plane);
+    const long c = k - (long)m * plane;
+    const int i = (int)(c / ny);
+    const double* rp = work;
+    const double* rm = work + total;
+    const double* a = work + 2 * total;
+    auto limited = [&](long kl, long kr) {   // face between cell kl and cell kr = kl + ny
+        const double f = a[kl];
+        return (f > 0.0 ? fmin(rp[kr], rm[kl]) : fmin(rp[kl], rm[kr])) * f;
+    };
+    const double fl = i > 0 ? limited(k - ny, k) : 0.0;
+    const double fr = i + 1 < (int)nx ? limited(k, k + ny) : 0.0;
+    real* u = (real*)st.p[m];
+    u[9 * plane + c] = real(double(u[9 * plane + c]) + (fl - fr));
+}
+
 
 
 }  // namespace
@@ -934,6 +999,39 @@ int amhd_closure_apply(void* const* states, const void* const* srcs, const doubl
         if (dtype == AMHD_F64) closure_apply_kernel<double><<<g, 256, 0, s>>>(st, sr, sd, cnt, plane);
         else closure_apply_kernel<float><<<g, 256, 0, s>>>(st, sr, sd, cnt, plane);
         if (hipGetLastError() != hipSuccess) return cfail(-2, "amhd_closure_apply: launch failed");
+    }
+    return 0;
+}
+
+int amhd_closure_sharpen(void* const* states, const void* const* bounds, int dtype, int n, int nx, int ny,
+                         double kappa, double* work, void* stream) {
+    if (!states || n < 0 || nx <= 0 || ny <= 0 || !(kappa >= 0.0) || !std::isfinite(kappa))
+        return cfail(-1, "amhd_closure_sharpen: bad argument");
+    if (dtype != AMHD_F64 && dtype != AMHD_F32) return cfail(-1, "amhd_closure_sharpen: bad dtype %d", dtype);
+    if (kappa == 0.0 || n == 0) return 0;
+    if (!work) return cfail(-1, "amhd_closure_sharpen: work is null");
+    const long plane = (long)nx * ny;
+    hipStream_t s = (hipStream_t)stream;
+    for (int m0 = 0; m0 < n; m0 += CHUNK) {
+        const int cnt = n - m0 < CHUNK ? n - m0 : CHUNK;
+        MutPtrs st{};
+        ConstPtrs cs{}, bs{};
+        for (int m = 0; m < cnt; ++m) {
+            if (!states[m0 + m] || (bounds && !bounds[m0 + m]))
+                return cfail(-1, "amhd_closure_sharpen: null pointer %d", m0 + m);
+            st.p[m] = states[m0 + m];
+            cs.p[m] = states[m0 + m];
+            bs.p[m] = bounds ? bounds[m0 + m] : nullptr;
+        }
+        const unsigned g = (unsigned)(((long)cnt * plane + 255) / 256);
+        if (dtype == AMHD_F64) {
+            sharpen_ratios_kernel<double><<<g, 256, 0, s>>>(cs, bs, bounds != nullptr, cnt, nx, ny, kappa, work);
+            sharpen_apply_kernel<double><<<g, 256, 0, s>>>(st, cnt, nx, ny, work);
+        } else {
+            sharpen_ratios_kernel<float><<<g, 256, 0, s>>>(cs, bs, bounds != nullptr, cnt, nx, ny, kappa, work);
+            sharpen_apply_kernel<float><<<g, 256, 0, s>>>(st, cnt, nx, ny, work);
+        }
+        if (hipGetLastError() != hipSuccess) return cfail(-2, "amhd_closure_sharpen: launch failed");
     }
     return 0;
 }

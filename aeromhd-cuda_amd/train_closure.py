@@ -4,8 +4,9 @@
         --iters 3000 --out aeromhd-cuda_amd/closure_weights.npz --report closure_report.json
 
 (the recipe of the shipped weights: the mixing flux alone, no aggregation
-rounds, the correction gain calibrated on training seeds 0-7), and to
-evaluate given weights (calibrating them first unless --calibrate ''):
+rounds, the correction gain and then the coefficient of the limited
+anti-diffusion of C calibrated on training seeds 0-7), and to evaluate given
+weights (calibrating them first unless --calibrate ''):
 
     python aeromhd-cuda_amd/train_closure.py --evaluate-only W.npz --out W_calibrated.npz
 
@@ -31,23 +32,26 @@ def log(*a):
 
 
 def final_evaluation(rep, model, eval_seeds, variants, args):
-    """Evaluate on the held-out seeds (with the limiter switched off as an
-    ablation), time the FP64 fine solve, write the report."""
+    """Evaluate on the held-out seeds (with ablations: the network without
+    the sharpening of C, the sharpening without the network, no limiters),
+    time the FP64 fine solve, write the report."""
     import amhd_closure as cl
 
-    rows = cl.evaluate(eval_seeds, model, steps=args.fine_steps, log=log, variants=variants + ("nolimit",))
+    extra = tuple(v for v in ("network", "sharpen", "nolimit") if v not in variants)
+    rows = cl.evaluate(eval_seeds, model, steps=args.fine_steps, log=log, variants=variants + extra)
     rep["evaluation"] = rows
     rep["fp64_fine"] = cl.fp64_fine_throughput(eval_seeds[0], args.fine_steps)
     fine_cells = 2048 * 1024 * args.fine_steps
-    tags = ("plain",) + variants + ("nolimit",)
+    tags = ("plain",) + variants + extra
     summ = cl.summarize(rows, tags)
     fp64 = rep["fp64_fine"]["mcups"]
     for t in tags:
         summ[f"speedup_{t}_vs_fp64_fine"] = fine_cells / summ[f"wall_{t}_s_mean"] / 1e6 / fp64
     bt = cl.evaluate_batched(eval_seeds, model, steps=args.fine_steps, log=log)
     fine_cells = 2048 * 1024 * args.fine_steps * len(eval_seeds)
-    for t in ("plain", "closure"):
-        bt[f"speedup_{t}_vs_fp64_fine"] = fine_cells / bt[f"wall_{t}_s"] / 1e6 / fp64
+    for t in ("plain", "closure", "sharpen"):
+        if f"wall_{t}_s" in bt:
+            bt[f"speedup_{t}_vs_fp64_fine"] = fine_cells / bt[f"wall_{t}_s"] / 1e6 / fp64
     rep["batched"] = bt
     rep["summary"] = summ
     log(f"[closure] summary {summ}")
@@ -75,6 +79,10 @@ def main():
     ap.add_argument("--evaluate-only", default="", help="evaluate these weights on the test seeds; no training")
     ap.add_argument("--calibrate", default="0:8", help="training seeds a:b for the gain calibration ('' to skip)")
     ap.add_argument("--gains", default="1.0,1.1,1.2,1.3,1.4,1.5")
+    ap.add_argument("--kappas", default="0.006,0.008,0.009,0.01,0.011,0.012,0.014",
+                    help="anti-diffusion coefficients tried after the gain ('' to skip)")
+    ap.add_argument("--kappas-alone", default="0.012,0.014,0.016,0.018,0.02,0.022,0.025",
+                    help="coefficients tried for the sharpening-alone ablation ('' to skip)")
     ap.add_argument("--out", default=os.path.join(os.path.dirname(os.path.abspath(__file__)), "closure_weights.npz"))
     ap.add_argument("--report", default="closure_report.json")
     args = ap.parse_args()
@@ -92,6 +100,11 @@ def main():
         ca, cb = (int(v) for v in args.calibrate.split(":"))
         g, err = cl.calibrate_gain(model, range(ca, cb), gains, steps=args.fine_steps, log=log)
         rep["calibration"] = {"seeds": args.calibrate, "mw_err_by_gain": err, "gain": g}
+        for key, grid, alone in (("kappa", args.kappas, False), ("kappa_alone", args.kappas_alone, True)):
+            if grid:
+                k, kerr = cl.calibrate_kappa(model, range(ca, cb), [float(v) for v in grid.split(",")],
+                                             steps=args.fine_steps, log=log, alone=alone)
+                rep["calibration"].update({f"mw_err_by_{key}": kerr, key: k})
         model.save(args.out)
 
     if args.evaluate_only:

@@ -324,6 +324,17 @@ int amhd_closure_check(const void* const* states, const void* const* srcs, const
                        int nx, int ny, double gamma, double dx, double dy, double* work, double* out, void* stream);
 int amhd_closure_apply(void* const* states, const void* const* srcs, const double* dt, int dtype, int n, int nx,
                        int ny, void* stream);
+/* Limited anti-diffusion of the concentration across x faces, one pass of
+ * flux-corrected transport (Zalesak 1979) on states[m] in place: face
+ * i+1/2 carries A = kappa * (rho_i + rho_{i+1}) / 2 * (C_{i+1} - C_i) of
+ * rhoC (x-boundary faces nothing), limited so that every cell keeps C within
+ * the min/max of C over the cell and its face neighbours, in states and,
+ * if bounds (n device addresses, or NULL) is given, in those states as
+ * well; rho and every other plane are untouched and rhoC is conserved row
+ * by row.  kappa >= 0 (0: nothing runs); work: device workspace of
+ * 3 * min(n, 32) * nx * ny doubles. */
+int amhd_closure_sharpen(void* const* states, const void* const* bounds, int dtype, int n, int nx, int ny,
+                         double kappa, double* work, void* stream);
 
 #ifdef __cplusplus
 }

@@ -245,6 +245,73 @@ def test_limited_source_reduces_and_conserves():
     assert np.all(co.limited_source(dF, zero, zero, fs, 0.3, 0.2, C) == 0.0)
 
 
+def _profile(c_row, ny=3, rho=None):
+    """One instance whose every y row holds the x profile c_row."""
+    nx = len(c_row)
+    U = np.zeros((1, 10, nx, ny))
+    U[0, 0] = 1.0 if rho is None else np.asarray(rho, dtype=np.float64)[:, None]
+    U[0, 9] = U[0, 0] * np.asarray(c_row, dtype=np.float64)[:, None]
+    return U
+
+
+@pytest.mark.parametrize("kappa,expect", [(0.1, [1.0, 0.8, 0.2, 0.0]), (1.0, [1.0, 1.0, 0.0, 0.0])])
+def test_sharpen_four_cell_profile(kappa, expect):
+    """rho = 1, C = (1, 0.75, 0.25, 0) along x.  The outer faces move
+    nothing (their flux would push C of the end cells, already at their
+    neighbourhood's max and min, beyond it); the middle face's flux
+    kappa * (0.25 - 0.75) = -kappa / 2 moves rhoC from cell 2 to cell 1.
+    Cell 1 may gain Q+ = 0.25 and its inflow is kappa / 2; cell 2 may lose
+    Q- = 0.25 and its outflow is kappa / 2.  kappa = 0.1: both ratios are
+    above 1, the whole 0.05 moves: (1, 0.8, 0.2, 0).  kappa = 1: both are
+    0.5, a quarter moves: the sharp step (1, 1, 0, 0)."""
+    U = _profile([1.0, 0.75, 0.25, 0.0])
+    out = co.sharpen_x(U, None, kappa)
+    np.testing.assert_allclose(out[0, 9] / out[0, 0], np.repeat(np.asarray(expect)[:, None], 3, axis=1),
+                               rtol=0, atol=1e-15)
+    # the mirrored profile sharpens the mirrored way
+    outm = co.sharpen_x(_profile([0.0, 0.25, 0.75, 1.0]), None, kappa)
+    np.testing.assert_allclose(outm[0, 9, :, 0], np.asarray(expect)[::-1], rtol=0, atol=1e-15)
+
+
+def test_sharpen_bounds_conservation_and_identity():
+    """Random states: C stays within the min / max over each cell and its
+    face neighbours (of the state, and of a second state when given), where
+    the unlimited fluxes (every ratio 1) leave them; rhoC is conserved in
+    every row (x faces only); every other plane is untouched; kappa = 0 is
+    the identity; a y-uniform state stays y-uniform; the pass narrows a
+    smeared interface (sum of C (1 - C) falls)."""
+    rng = np.random.default_rng(41)
+    U = _state(rng, 2, 12, 7, 5.0 / 3.0)
+    Ub = _state(rng, 2, 12, 7, 5.0 / 3.0)
+    for kappa in (0.05, 0.5, 5.0):
+        for b in (None, Ub):
+            out = co.sharpen_x(U, b, kappa)
+            C1 = out[:, 9] / out[:, 0]
+            lo, hi = co.local_bounds(co.concentration(U))
+            if b is not None:
+                blo, bhi = co.local_bounds(co.concentration(b))
+                lo, hi = np.minimum(lo, blo), np.maximum(hi, bhi)
+            assert np.all(C1 >= lo - 1e-13) and np.all(C1 <= hi + 1e-13)
+            np.testing.assert_allclose(out[:, 9].sum(axis=1), U[:, 9].sum(axis=1), rtol=1e-13)
+            np.testing.assert_array_equal(np.delete(out, 9, axis=1), np.delete(U, 9, axis=1))
+            assert np.any(out[:, 9] != U[:, 9])
+    # the unlimited fluxes break the bounds at kappa = 5
+    C = co.concentration(U)
+    A = 5.0 * 0.5 * (U[:, 0, :-1] + U[:, 0, 1:]) * (C[:, 1:] - C[:, :-1])
+    rc = U[:, 9].copy()
+    rc[:, :-1] -= A
+    rc[:, 1:] += A
+    lo, hi = co.local_bounds(C)
+    assert np.any(rc / U[:, 0] > hi + 1e-6) or np.any(rc / U[:, 0] < lo - 1e-6)
+    np.testing.assert_array_equal(co.sharpen_x(U, Ub, 0.0), U)
+    x = np.arange(16)
+    Up = _profile(0.5 * (1 - np.tanh((x - 7.5) / 2.5)), ny=5, rho=1 + x / 16)
+    out = co.sharpen_x(Up, None, 0.2)
+    assert np.all(out[0, 9] == out[0, 9, :, :1])
+    c0, c1 = co.concentration(Up)[0, :, 0], co.concentration(out)[0, :, 0]
+    assert (c1 * (1 - c1)).sum() < (c0 * (1 - c0)).sum()
+
+
 def test_donor_cell_flux():
     """F_rhoC = C_donor F_rho + G with the donor upstream of the mass flux."""
     fx = np.zeros((1, 3, 2, 9))

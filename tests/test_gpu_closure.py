@@ -223,12 +223,72 @@ def test_closure_glue_vs_oracle(torch_gpu, dtype):
     assert np.all(c1 >= lo - 10 * tol) and np.all(c1 <= hi + 10 * tol)
 
 
+@pytest.mark.parametrize("dtype", [0, 1])
+def test_closure_sharpen_vs_oracle(torch_gpu, dtype):
+    """amhd_closure_sharpen on 35 random states (two launches of up to 32
+    instances), with and without the second state's bounds, at a kappa that
+    leaves some faces unlimited and one that saturates them: equal to the
+    oracle (FP64: 1e-12 of max|rhoC|, FP32 states: 1e-6), C within the
+    bounds, rhoC conserved row by row, every other plane bit for bit as
+    before, kappa = 0 a no-op."""
+    torch = torch_gpu
+    import amhd_lib as L
+    from test_closure_oracle import _state
+
+    lib = L.load()
+    rng = np.random.default_rng(50 + dtype)
+    gamma = 5.0 / 3.0
+    n, nx, ny = 35, 12, 10
+    tdt = torch.float64 if dtype == 0 else torch.float32
+    U0 = _state(rng, n, nx, ny, gamma)
+    U0[:, 9] = U0[:, 0] * np.clip(rng.uniform(-0.3, 1.3, (n, nx, ny)), 0, 1)   # pure and mixed cells
+    Ub = torch.tensor(_state(rng, n, nx, ny, gamma), dtype=tdt, device="cuda").contiguous()
+    Ubh = Ub.cpu().numpy().astype(np.float64)
+    esz = 8 if dtype == 0 else 4
+    work = torch.empty(3 * 32 * nx * ny, device="cuda", dtype=torch.float64)
+    V = C.c_void_p * n
+    tol = 1e-12 if dtype == 0 else 1e-6
+    for kappa in (0.05, 2.0):
+        for with_b in (False, True):
+            Ud = torch.tensor(U0, dtype=tdt, device="cuda").contiguous()
+            Uh = Ud.cpu().numpy().astype(np.float64)
+            ptrs = V(*[Ud.data_ptr() + m * 10 * nx * ny * esz for m in range(n)])
+            bptrs = V(*[Ub.data_ptr() + m * 10 * nx * ny * esz for m in range(n)]) if with_b else None
+            print(f"sharpen dtype {dtype} kappa {kappa} bounds {with_b}", flush=True)
+            L.check(lib.amhd_closure_sharpen(ptrs, bptrs, dtype, n, nx, ny, kappa, _p(work), L.stream_handle(torch)),
+                    "amhd_closure_sharpen")
+            torch.cuda.synchronize()
+            out = Ud.cpu().numpy().astype(np.float64)
+            ref = co.sharpen_x(Uh, Ubh if with_b else None, kappa)
+            scale = np.abs(Uh[:, 9]).max()
+            np.testing.assert_allclose(out[:, 9], ref[:, 9], rtol=0, atol=tol * scale)
+            assert np.any(np.abs(out[:, 9] - Uh[:, 9]) > 100 * tol * scale)
+            np.testing.assert_array_equal(np.delete(out, 9, axis=1), np.delete(Uh, 9, axis=1))
+            np.testing.assert_allclose(out[:, 9].sum(axis=1), Uh[:, 9].sum(axis=1), rtol=0,
+                                       atol=10 * tol * np.abs(Uh[:, 9]).sum(axis=1).max())
+            lo, hi = co.local_bounds(co.concentration(Uh))
+            if with_b:
+                blo, bhi = co.local_bounds(co.concentration(Ubh))
+                lo, hi = np.minimum(lo, blo), np.maximum(hi, bhi)
+            c1 = out[:, 9] / out[:, 0]
+            assert np.all(c1 >= lo - 10 * tol) and np.all(c1 <= hi + 10 * tol)
+    Ud = torch.tensor(U0, dtype=tdt, device="cuda").contiguous()
+    before = Ud.cpu().numpy()
+    ptrs = V(*[Ud.data_ptr() + m * 10 * nx * ny * esz for m in range(n)])
+    L.check(lib.amhd_closure_sharpen(ptrs, None, dtype, n, nx, ny, 0.0, None, L.stream_handle(torch)),
+            "amhd_closure_sharpen")
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(Ud.cpu().numpy(), before)
+
+
 def test_closure_gain_over_plain_coarse(torch_gpu):
     """The shipped network (closure_weights.npz, which applies the mixing
-    flux alone, with its calibrated gain) against the plain coarse solver on
-    two held-out seeds, measured against the fine 2048x1024 solver
-    restricted to the coarse grid: the mixing-width error under a fifth of
-    the plain solver's, the density (which the mixing flux cannot change) as
+    flux alone, with its calibrated gain, then the sharpening of C at its
+    calibrated coefficient) against the plain coarse solver on two held-out
+    seeds, measured against the fine 2048x1024 solver restricted to the
+    coarse grid: the mixing-width error within 5% and under a fifth of the
+    plain solver's, the concentration error below the plain solver's, the
+    density (which neither the mixing flux nor the sharpening changes) as
     the plain solver's, no fallbacks."""
     import amhd_closure as cl
 
@@ -239,6 +299,8 @@ def test_closure_gain_over_plain_coarse(torch_gpu):
     rows = cl.evaluate([210, 217], model, log=print)
     for r in rows:
         assert r["mw_err_closure"] < 0.2 * r["mw_err_plain"], r
+        assert r["mw_err_closure"] < 0.05, r
+        assert r["l1c_closure"] < r["l1c_plain"], r
         assert r["l1rho_closure"] == pytest.approx(r["l1rho_plain"], rel=1e-6), r
         assert r["fallbacks_closure"] == 0, r
 
@@ -249,6 +311,37 @@ def _coarse_sieve(seed):
     cfg, W = magnetic_sieve(nx=256, ny=128, precision="fp32", seed=seed, max_steps=10 ** 6)
     cfg.t_end = 1e30
     return cfg, cons_from_prim_host(W, cfg.gamma)
+
+
+def test_sharpening_step_is_the_plain_step_then_the_oracle(torch_gpu):
+    """One host-driven step of the sharpening alone (ClosureSolver without a
+    network, kappa > 0) equals the plain step of the same dt from the same
+    state followed by the oracle's sharpen_x on the post-step state with the
+    pre-step state's bounds (FP32 states: 1e-6 of max|rhoC|); only rhoC
+    differs from the plain step, and it does differ."""
+    import amhd_closure as cl
+
+    cfg, U = _coarse_sieve(216)
+    kappa = 0.3
+    post = {}
+    for tag, kw in (("plain", {}), ("sharp", {"kappa": kappa})):
+        cs = cl.ClosureSolver(cfg, None, **kw)
+        cs.s.U = U.copy()
+        cs.begin()
+        pre = cs.s.ctx.download_numpy()
+        dt = 0.5 * cs.s.compute_dt()
+        print(f"sharpening step {tag}: dt {dt:.6e}", flush=True)
+        assert cs.step(dt) is True
+        post[tag] = (pre, dt, cs.s.ctx.download_numpy())
+    (pre_p, dt_p, post_p), (pre_s, dt_s, post_s) = post["plain"], post["sharp"]
+    np.testing.assert_array_equal(pre_p, pre_s)
+    assert dt_p == dt_s
+    inner = (slice(None), slice(2, -2), slice(2, -2))
+    ref = co.sharpen_x(post_p[inner][None].astype(np.float64), pre_p[inner][None].astype(np.float64), kappa)[0]
+    scale = np.abs(post_p[9][2:-2, 2:-2]).max()
+    np.testing.assert_allclose(post_s[9][2:-2, 2:-2], ref[9], rtol=0, atol=1e-6 * scale)
+    assert np.any(np.abs(post_s[9][2:-2, 2:-2] - post_p[9][2:-2, 2:-2]) > 1e-4 * scale)
+    np.testing.assert_array_equal(np.delete(post_s[inner], 9, axis=0), np.delete(post_p[inner], 9, axis=0))
 
 
 def test_closure_fallback_is_the_plain_step(torch_gpu):
@@ -287,13 +380,20 @@ def test_closure_guard_catches_bad_corrections(torch_gpu):
     assert nonfinite == 0 and rho_min > 0 and p_min > 0
 
 
-def test_batched_closure_equals_single_instances(torch_gpu):
+@pytest.mark.parametrize("form", ["closure", "sharpen"])
+def test_batched_closure_equals_single_instances(torch_gpu, form):
     """Three held-out seeds in one BatchClosure (one batched step, one
-    network pass, one batched correction per step) end bit for bit where
-    ClosureSolver takes each alone, with the same step counts."""
+    network pass, one batched correction and sharpening per step; or, for
+    the sharpening alone, the batched step and one batched sharpening) end
+    bit for bit where ClosureSolver takes each alone, with the same step
+    counts."""
     import amhd_closure as cl
 
     model = cl.FluxCNN.load(os.path.join(PKG, "closure_weights.npz"))
+    kw = {}
+    if form == "sharpen":
+        kw = {"kappa": float(model.stats["kappa_alone"])}
+        model = None
     cases = []
     for seed, t_end in ((213, 0.004), (214, 0.0035), (215, 0.003)):
         cfg, U = _coarse_sieve(seed)
@@ -301,11 +401,11 @@ def test_batched_closure_equals_single_instances(torch_gpu):
         cases.append((cfg, U))
     singles = []
     for cfg, U in cases:
-        cs = cl.ClosureSolver(cfg, model)
+        cs = cl.ClosureSolver(cfg, model, **kw)
         cs.s.U = U.copy()
         cs.run(cfg.t_end)
         singles.append((cs.s.ctx.download_numpy(), cs.s.step, cs.s.t))
-    bc = cl.BatchClosure(cases, model)
+    bc = cl.BatchClosure(cases, model, **kw)
     bc.run()
     for k, (U1, steps, t) in enumerate(singles):
         print(f"instance {k}: steps {bc.b.steps[k]} / {steps}, fallbacks {bc.fallbacks[k]}", flush=True)
