@@ -666,6 +666,13 @@ def _dvec(v):
     return (C.c_double * len(v))(*v)
 
 
+def check_kappa(kappa) -> None:
+    """The sharpening coefficient: None (the model's) or a finite number >= 0."""
+    if kappa is not None and not (isinstance(kappa, (int, float, np.floating)) and math.isfinite(kappa)
+                                  and kappa >= 0):
+        raise ValueError(f"kappa must be None or a finite number >= 0, got {kappa!r}")
+
+
 def check_closure_boundaries(cfg) -> None:
     """The closure's kernels (features, face corrections, limiters, guard)
     wrap y periodically and carry no correction through the x boundaries,
@@ -696,6 +703,7 @@ class ClosureSolver:
         from solver import MHDSolver
 
         check_closure_boundaries(cfg)
+        check_kappa(kappa)
         self.s = MHDSolver(cfg)
         self.model = model
         self.fallbacks = 0
@@ -779,6 +787,7 @@ class BatchClosure:
 
     def __init__(self, cases, model: FluxCNN | None, divb_floor: float = 1.0, keep: float | None = 0.5,
                  fields=None, gain: float | None = None, kappa: float | None = None):
+        check_kappa(kappa)
         torch = _L.require_gpu()
         from amhd_batch import BatchSolver
 

@@ -289,18 +289,20 @@ def test_closure_gain_over_plain_coarse(torch_gpu):
     coarse grid: the mixing-width error within 5% and under a fifth of the
     plain solver's, the concentration error below the plain solver's, the
     density (which neither the mixing flux nor the sharpening changes) as
-    the plain solver's, no fallbacks."""
+    the plain solver's, no fallbacks; the sharpening alone (no network, its
+    own calibrated coefficient) within 5% as well."""
     import amhd_closure as cl
 
     path = os.path.join(PKG, "closure_weights.npz")
     assert os.path.exists(path), "closure_weights.npz (trained weights) missing"
     model = cl.FluxCNN.load(path)
     assert cl.applied_fields(model.stats) == [cl.RC]
-    rows = cl.evaluate([210, 217], model, log=print)
+    rows = cl.evaluate([210, 217], model, log=print, variants=("closure", "sharpen"))
     for r in rows:
         assert r["mw_err_closure"] < 0.2 * r["mw_err_plain"], r
         assert r["mw_err_closure"] < 0.05, r
         assert r["l1c_closure"] < r["l1c_plain"], r
+        assert r["mw_err_sharpen"] < 0.05, r
         assert r["l1rho_closure"] == pytest.approx(r["l1rho_plain"], rel=1e-6), r
         assert r["fallbacks_closure"] == 0, r
 

@@ -61,6 +61,42 @@ def test_closure_source_checks_arguments_before_device_work():
     assert b"rho" in lib.amhd_last_error()
 
 
+def test_closure_sharpen_checks_arguments_before_device_work():
+    """amhd_closure_sharpen refuses a negative, NaN or infinite kappa, a bad
+    dtype, a null state list and, with kappa > 0, a null workspace -- before
+    any launch; kappa = 0 returns at once with nothing to do.  The host-side
+    solvers refuse such a kappa too; no device needed."""
+    import numpy as np
+    import amhd_closure as cl
+    import amhd_lib
+
+    lib = amhd_lib.load()
+    buf = np.zeros(64, dtype=np.float32)
+    st = (C.c_void_p * 1)(buf.ctypes.data)
+    work = C.c_void_p(buf.ctypes.data)
+
+    def call(states, dtype, kappa, w):
+        return lib.amhd_closure_sharpen(states, None, dtype, 1, 4, 4, kappa, w, None)
+
+    for k in (-1.0, float("nan"), float("inf")):
+        assert call(st, amhd_lib.AMHD_F32, k, work) == -1
+        assert b"bad argument" in lib.amhd_last_error()
+    assert call(st, 7, 0.1, work) == -1
+    assert b"dtype" in lib.amhd_last_error()
+    assert call(None, amhd_lib.AMHD_F32, 0.1, work) == -1
+    assert call(st, amhd_lib.AMHD_F32, 0.1, None) == -1
+    assert b"work" in lib.amhd_last_error()
+    assert call(st, amhd_lib.AMHD_F32, 0.0, None) == 0
+    from amhd_ics import magnetic_sieve
+
+    cfg, _ = magnetic_sieve(nx=256, ny=128, precision="fp32", seed=0)
+    for k in (-0.5, float("nan")):
+        with pytest.raises(ValueError, match="kappa"):
+            cl.ClosureSolver(cfg, None, kappa=k)
+        with pytest.raises(ValueError, match="kappa"):
+            cl.BatchClosure([(cfg, None)], None, kappa=k)
+
+
 def test_params_struct_layout_matches_header():
     import amhd_lib
 
