@@ -1037,7 +1037,8 @@ def evaluate_resolutions(seeds, factors=(4, 2), steps: int = 2000, nx: int = 204
     initial states restricted by f advance as one plain FP32 batch to their
     fine runs' end times; the result, restricted to the closure's grid
     (FACTOR), is scored as evaluate_batched scores the closure (integral
-    mixing width and L1 density error against the restricted fine state).
+    mixing width, L1 density error and mean |C - C_fine| against the
+    restricted fine state).
     Wall time as in evaluate_batched: the median of `reps` timed runs after
     an untimed 2-step run.  This gives the plain solver's cost of reaching a
     given error, against which the closure's cost at its error compares."""
@@ -1055,7 +1056,9 @@ def evaluate_resolutions(seeds, factors=(4, 2), steps: int = 2000, nx: int = 204
         s = fine_run(seed, nx, ny, steps, grab0, [0])
         Uf = s.ctx.download()
         ccfg = coarse_config(s.cfg)
-        refs.append((restrict(Uf)[0].cpu().numpy(), mixing_width(ghosted(restrict(Uf)).cpu().numpy(), ccfg), ccfg))
+        Ufr = restrict(Uf).cpu().numpy()
+        refs.append((Ufr[0], mixing_width(ghosted(restrict(Uf)).cpu().numpy(), ccfg), ccfg,
+                     np.clip(Ufr[9] / Ufr[0], 0, 1)))
         fines.append((s.cfg, fine["U0"], s.t))
     out = {"seeds": list(seeds)}
     for f in factors:
@@ -1079,16 +1082,17 @@ def evaluate_resolutions(seeds, factors=(4, 2), steps: int = 2000, nx: int = 204
                 pass
             torch.cuda.synchronize()
             walls.append(time.perf_counter() - t0)
-        mw_err, l1 = [], []
-        for k, (rho_f, mw_fr, ccfg) in enumerate(refs):
+        mw_err, l1, l1c = [], [], []
+        for k, (rho_f, mw_fr, ccfg, c_f) in enumerate(refs):
             Uc = restrict(torch.from_numpy(b.download(k)), FACTOR // f).numpy()
             mw_err.append(abs(mixing_width(ghosted(torch.from_numpy(Uc).cuda()).cpu().numpy(), ccfg) - mw_fr)
                           / max(abs(mw_fr), 1e-30))
             l1.append(float(np.abs(Uc[0] - rho_f).mean() / np.abs(rho_f).mean()))
+            l1c.append(float(np.abs(np.clip(Uc[9] / Uc[0], 0, 1) - c_f).mean()))
         tag = f"{nx // f}x{ny // f}"
         out[tag] = {"factor": f, "wall_s": float(np.median(walls)), "wall_runs_s": [round(w, 4) for w in walls],
                     "mw_err_mean": float(np.mean(mw_err)), "l1rho_mean": float(np.mean(l1)),
-                    "steps": int(b.steps.sum())}
+                    "l1c_mean": float(np.mean(l1c)), "steps": int(b.steps.sum())}
         if log:
             log(f"[closure] plain {tag}: " + ", ".join(f"{k}={v}" for k, v in out[tag].items()))
     return out
