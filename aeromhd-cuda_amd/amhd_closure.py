@@ -853,7 +853,8 @@ class BatchClosure:
 # ------------------------------------------------------- evaluation
 VARIANTS = {
     "closure": dict(keep=0.5),                       # the model's own fields, gain and kappa
-    "network": dict(keep=0.5, kappa=0.0),            # ... without the sharpening of C
+    "network": dict(keep=0.5, kappa=0.0, gain="alone"),   # ... without the sharpening of C, at the
+                                                          # network's own gain (stats["gain_alone"])
     "nolimit": dict(keep=None, kappa=0.0),           # ... and without the limiters
     # the sharpening alone (no network pass) at its own calibrated
     # coefficient (stats["kappa_alone"])
@@ -868,11 +869,15 @@ VARIANTS = {
 
 def _resolve(model, kw):
     """(model or None, ClosureSolver keywords) of a variant: kappa="alone"
-    becomes the model's calibrated coefficient of the sharpening alone, and
-    network=False drops the model."""
+    becomes the model's calibrated coefficient of the sharpening alone,
+    gain="alone" the gain calibrated for the network without the sharpening
+    (the closure's own gain if none is stored), and network=False drops the
+    model."""
     kw = dict(kw)
     if kw.get("kappa") == "alone":
         kw["kappa"] = float(np.asarray(model.stats.get("kappa_alone", 0.0)))
+    if kw.get("gain") == "alone":
+        kw["gain"] = float(np.asarray(model.stats.get("gain_alone", model.stats.get("gain", 1.0))))
     return (model if kw.pop("network", True) else None), kw
 
 
@@ -1098,12 +1103,14 @@ def evaluate_resolutions(seeds, factors=(4, 2), steps: int = 2000, nx: int = 204
     return out
 
 
-def calibrate_gain(model, seeds, gains, steps: int = 2000, log=None):
+def calibrate_gain(model, seeds, gains, steps: int = 2000, log=None, kappa=None):
     """The correction gain that minimises the mean mixing-width error over
     `seeds` (training seeds: the held-out ones stay untouched); a network
     fitted by least squares under-predicts sharp corrections, and a scalar
-    gain is the one-parameter remedy.  Stores it in model.stats["gain"]."""
-    specs = {f"g{g:g}": dict(keep=0.5, gain=float(g)) for g in gains}
+    gain is the one-parameter remedy.  kappa: the sharpening coefficient
+    during the calibration (None: the model's).  Stores the gain in
+    model.stats["gain"]."""
+    specs = {f"g{g:g}": dict(keep=0.5, gain=float(g), kappa=kappa) for g in gains}
     rows = evaluate(seeds, model, steps=steps, variants=tuple(specs), specs=specs)
     err = {t: sum(r[f"mw_err_{t}"] for r in rows) / len(rows) for t in specs}
     best = min(err, key=err.get)
@@ -1130,6 +1137,23 @@ def calibrate_kappa(model, seeds, kappas, steps: int = 2000, log=None, alone: bo
         log(f"[closure] {'sharpening-alone ' if alone else ''}kappa calibration on seeds {list(seeds)}: " +
             ", ".join(f"{t}={e:.4f}" for t, e in err.items()) + f" -> {best}")
     return float(specs[best]["kappa"]), err
+
+
+def calibrate_gain_kappa(model, seeds, gains, kappas, steps: int = 2000, log=None):
+    """The (gain, kappa) pair of the grid gains x kappas that minimises the
+    mean mixing-width error over `seeds` (training seeds): the gain scales
+    the network's mixing flux and kappa the sharpening that follows it, and
+    the two trade against each other.  Stores both in model.stats."""
+    specs = {f"g{g:g}_k{k:g}": dict(keep=0.5, gain=float(g), kappa=float(k)) for g in gains for k in kappas}
+    rows = evaluate(seeds, model, steps=steps, variants=tuple(specs), specs=specs)
+    err = {t: sum(r[f"mw_err_{t}"] for r in rows) / len(rows) for t in specs}
+    best = min(err, key=err.get)
+    model.stats["gain"] = np.asarray(specs[best]["gain"], dtype=np.float32)
+    model.stats["kappa"] = np.asarray(specs[best]["kappa"], dtype=np.float64)
+    if log:
+        log(f"[closure] gain x kappa calibration on seeds {list(seeds)}: " +
+            ", ".join(f"{t}={e:.4f}" for t, e in err.items()) + f" -> {best}")
+    return float(specs[best]["gain"]), float(specs[best]["kappa"]), err
 
 
 def summarize(rows, tags) -> dict:

@@ -4,9 +4,9 @@
         --iters 3000 --out aeromhd-cuda_amd/closure_weights.npz --report closure_report.json
 
 (the recipe of the shipped weights: the mixing flux alone, no aggregation
-rounds, the correction gain and then the coefficient of the limited
-anti-diffusion of C calibrated on training seeds 0-7), and to evaluate given
-weights (calibrating them first unless --calibrate ''):
+rounds, the correction gain and the coefficient of the limited
+anti-diffusion of C calibrated together on training seeds 0-7), and to
+evaluate given weights (calibrating them first unless --calibrate ''):
 
     python aeromhd-cuda_amd/train_closure.py --evaluate-only W.npz --out W_calibrated.npz
 
@@ -23,6 +23,8 @@ import json
 import os
 import sys
 import time
+
+import numpy as np
 
 sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
 
@@ -78,9 +80,10 @@ def main():
     ap.add_argument("--fields", default="8", help="network fields the closure corrects (8 = the mixing flux)")
     ap.add_argument("--evaluate-only", default="", help="evaluate these weights on the test seeds; no training")
     ap.add_argument("--calibrate", default="0:8", help="training seeds a:b for the gain calibration ('' to skip)")
-    ap.add_argument("--gains", default="1.0,1.1,1.2,1.3,1.4,1.5")
-    ap.add_argument("--kappas", default="0.006,0.008,0.009,0.01,0.011,0.012,0.014",
-                    help="anti-diffusion coefficients tried after the gain ('' to skip)")
+    ap.add_argument("--gains", default="0.25,0.5,0.75,1.0,1.25,1.5,1.75")
+    ap.add_argument("--kappas", default="0.008,0.009,0.01,0.011,0.012,0.014",
+                    help="anti-diffusion coefficients, calibrated with the gain on the grid gains x kappas "
+                         "('' : the gain alone, no sharpening)")
     ap.add_argument("--kappas-alone", default="0.012,0.014,0.016,0.018,0.02,0.022,0.025",
                     help="coefficients tried for the sharpening-alone ablation ('' to skip)")
     ap.add_argument("--out", default=os.path.join(os.path.dirname(os.path.abspath(__file__)), "closure_weights.npz"))
@@ -98,13 +101,22 @@ def main():
         if not args.calibrate:
             return
         ca, cb = (int(v) for v in args.calibrate.split(":"))
-        g, err = cl.calibrate_gain(model, range(ca, cb), gains, steps=args.fine_steps, log=log)
-        rep["calibration"] = {"seeds": args.calibrate, "mw_err_by_gain": err, "gain": g}
-        for key, grid, alone in (("kappa", args.kappas, False), ("kappa_alone", args.kappas_alone, True)):
-            if grid:
-                k, kerr = cl.calibrate_kappa(model, range(ca, cb), [float(v) for v in grid.split(",")],
-                                             steps=args.fine_steps, log=log, alone=alone)
-                rep["calibration"].update({f"mw_err_by_{key}": kerr, key: k})
+        # the network alone (no sharpening): its own gain, kept for the ablation
+        ga, gerr = cl.calibrate_gain(model, range(ca, cb), gains, steps=args.fine_steps, log=log, kappa=0.0)
+        model.stats["gain_alone"] = np.asarray(ga, dtype=np.float32)
+        rep["calibration"] = {"seeds": args.calibrate, "mw_err_by_gain_alone": gerr, "gain_alone": ga}
+        if args.kappas:
+            g, k, err = cl.calibrate_gain_kappa(model, range(ca, cb), gains,
+                                                [float(v) for v in args.kappas.split(",")],
+                                                steps=args.fine_steps, log=log)
+            rep["calibration"].update({"mw_err_by_gain_kappa": err, "gain": g, "kappa": k})
+        else:
+            model.stats["kappa"] = np.asarray(0.0)
+            rep["calibration"].update({"gain": ga, "kappa": 0.0})
+        if args.kappas_alone:
+            k, kerr = cl.calibrate_kappa(model, range(ca, cb), [float(v) for v in args.kappas_alone.split(",")],
+                                         steps=args.fine_steps, log=log, alone=True)
+            rep["calibration"].update({"mw_err_by_kappa_alone": kerr, "kappa_alone": k})
         model.save(args.out)
 
     if args.evaluate_only:
